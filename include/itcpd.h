@@ -1,0 +1,166 @@
+/*
+ * itcpd.h — C ABI of libitcpd_hip.so, an MI355X (gfx950) engine for the
+ * normal-equation CP-ALS path of ITensorCPD.jl (dense tensors).
+ *
+ * The reference has no C ABI on this path: every extension point is a Julia
+ * dispatch point.  Each entry point below states which reference interface it
+ * replaces (paths relative to the ITensorCPD.jl source tree).  Conventions:
+ *
+ *  - Layout is Julia's: column-major, mode 1 fastest.  T(i1,...,iN) sits at
+ *    i1 + I1*(i2 + I2*(...)).  A factor A_n is I_n x R column-major, lambda is a
+ *    length-R vector, Grams are R x R.  Modes are 0-based in this ABI.
+ *  - Every function returns an itcpd_status; none aborts the host.  The text of
+ *    the last error of the calling thread is itcpd_last_error().
+ *  - Buffers are caller-owned.  Pointers tagged ITCPD_MEM_HOST are host
+ *    memory, ITCPD_MEM_DEVICE are device (HBM) pointers on the plan's device.
+ *    A device tensor handed to itcpd_plan_create is borrowed (not copied) and
+ *    must stay alive and unchanged for the plan's lifetime.
+ *  - A plan is not re-entrant: one thread at a time.  Different plans may run
+ *    concurrently; each owns its own HIP stream.
+ */
+#ifndef ITCPD_H
+#define ITCPD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ITCPD_OK = 0,
+    ITCPD_E_ARG = -1,          /* bad argument (shape, rank, mode, pointer) */
+    ITCPD_E_HIP = -2,          /* a HIP runtime call failed */
+    ITCPD_E_OOM = -3,          /* device allocation failed */
+    ITCPD_E_NAN = -4,          /* fit became NaN: mirrors throw("Error NAN"), converge_checks/fit_check.jl:40-42 */
+    ITCPD_E_STATE = -5,        /* call out of order (e.g. run before set_cpd) */
+    ITCPD_E_UNSUPPORTED = -6   /* valid request this build does not implement */
+} itcpd_status;
+
+/* Arithmetic of the MTTKRP GEMMs.  Grams, solves and the fit are FP64 always. */
+typedef enum { ITCPD_FP64 = 0, ITCPD_FP32 = 1, ITCPD_BF16X3 = 2 } itcpd_prec;
+
+/* The reference's MttkrpAlgorithm subtypes on this path
+ * (algorithms/als_algorithms/standard/tensor.jl:10-49).  Both compute the same
+ * MTTKRP; here both run the dimension-tree kernels. */
+typedef enum { ITCPD_ALG_KRP_FREE_NORMAL = 0, ITCPD_ALG_KRP_NORMAL = 1 } itcpd_alg;
+
+typedef enum { ITCPD_MEM_HOST = 0, ITCPD_MEM_DEVICE = 1 } itcpd_mem;
+
+typedef enum { ITCPD_CHECK_NONE = 0, ITCPD_CHECK_FIT = 1 } itcpd_check_kind;
+
+/* Convergence-check state, in/out.  Mirrors the mutable fields of
+ * FitCheck (converge_checks/fit_check.jl:5-20) and NoCheck (no_check.jl:1-6). */
+typedef struct {
+    int32_t kind;          /* itcpd_check_kind */
+    int32_t iter;          /* check.iter (in/out) */
+    int32_t counter;       /* FitCheck.counter (in/out) */
+    int32_t max_counter;   /* check.max_counter */
+    int32_t total_iter;    /* FitCheck.total_iter (out) */
+    int32_t sweeps_run;    /* out: sweeps executed by this itcpd_run call */
+    int32_t converged;     /* out: 1 if check_converge returned true */
+    int32_t reserved;
+    double tolerance;      /* FitCheck.tolerance */
+    double ref_norm;       /* FitCheck.ref_norm = norm(T) */
+    double lastfit;        /* check.lastfit (in/out) */
+    double final_fit;      /* FitCheck.final_fit (out); CPDFit(check) */
+} itcpd_check;
+
+typedef struct itcpd_plan itcpd_plan;
+
+/* Per-plan timing of the MTTKRP GEMM kernels, measured with HIP events on the
+ * plan's stream when profiling is on (itcpd_plan_set_profiling). */
+typedef struct {
+    int64_t launches;      /* GEMM launches timed */
+    double total_ms;       /* sum of their durations */
+    double flops;          /* algorithmic flops of those launches (2 * prod(I) * R each) */
+    double bytes;          /* algorithmic HBM bytes of those launches (T read once each) */
+} itcpd_kernel_stats;
+
+/* Replaces compute_als(target::ITensor, cp; alg, check, maxiter)
+ * (optimizers/als_optimizers/als_optimizer.jl:37-55) and the ALS struct (:5-10):
+ * binds the target tensor and allocates the per-mode caches on the device.
+ * elt_bytes: 8 (Float64 tensor) or 4 (Float32 tensor).  tensor_mem: where
+ * `tensor` lives; a host tensor is copied to the device once. */
+int itcpd_plan_create(itcpd_plan** plan, int32_t order, const int64_t* dims, int64_t rank,
+                      int32_t elt_bytes, const void* tensor, int32_t tensor_mem,
+                      int32_t prec, int32_t alg, int32_t device);
+
+int itcpd_plan_destroy(itcpd_plan* plan);
+
+/* Replaces building CPD{ITensor}(factors, lambda) (cpd.jl:7-11) as the optimiser's
+ * starting point: factors[n] is I_n x R column-major FP64, lambda length R.  Also
+ * seeds the Gram cache A_n^T A_n (optimizers/als_optimizers/standard/tensor.jl:12). */
+int itcpd_set_cpd(itcpd_plan* plan, const double* const* factors, const double* lambda,
+                  int32_t mem);
+
+/* Reads the current CPD back (the CPD returned by optimize.jl:34). */
+int itcpd_get_cpd(itcpd_plan* plan, double* const* factors, double* lambda, int32_t mem);
+
+/* Replaces random_CPD(target, rank; rng) (cpd.jl:48-82): per mode an I_n x R
+ * matrix of N(0,1) draws (Philox-4x32-10 counter stream keyed by `seed`,
+ * Box-Muller), columns normalised by row_norm (math_tools/row_norm.jl:4-24),
+ * lambda from the last mode; then seeds the Gram cache as itcpd_set_cpd. */
+int itcpd_random_cpd(itcpd_plan* plan, uint64_t seed);
+
+/* Replaces random_CPD(target, rank; rng) (cpd.jl:63-82) when only the target's
+ * indices are known: the same Philox-seeded draws and row_norm as
+ * itcpd_random_cpd, written to caller buffers (factors[n]: I_n x R column-major,
+ * lambda: R).  No tensor or plan is needed. */
+int itcpd_random_factors(int32_t order, const int64_t* dims, int64_t rank, uint64_t seed,
+                         double* const* factors, double* lambda, int32_t mem, int32_t device);
+
+/* Replaces optimize(cp, als) (optimizers/als_optimizers/optimize.jl:6-35): runs
+ * sweeps on the device until the check stops it.  fit_trace (host, nullable)
+ * receives one fit per executed sweep (FitCheck only). */
+int itcpd_run(itcpd_plan* plan, itcpd_check* check, double* fit_trace);
+
+/* ---- per-hook entry points (the same kernels that itcpd_run launches) ---- */
+
+/* compute_krp(::MttkrpAlgorithm, ...) (algorithms/als_algorithms/standard/
+ * MttkrpAlgorithm.jl:18-31): V = Hadamard product of the Grams of all modes != mode. */
+int itcpd_compute_krp(itcpd_plan* plan, int32_t mode, double* V_out, int32_t mem);
+
+/* matricize_tensor(::KRPFreeNormal / ::KRPNormal, ...) (algorithms/als_algorithms/
+ * standard/tensor.jl:12-20,32-44): M_n = T_(n) (Khatri-Rao of the current factors). */
+int itcpd_mttkrp(itcpd_plan* plan, int32_t mode, double* M_out, int32_t mem);
+
+/* solve_ls_problem(::MttkrpAlgorithm, ...) (MttkrpAlgorithm.jl:34-41) ->
+ * ldiv_solve!! (algebra/ldiv_solve.jl:4-29): solves V X^T = M^T with pivoted
+ * Cholesky (tol cholesky_epsilon = 1e-6, ITensorCPD.jl:2) and falls back to the
+ * column-pivoted QR minimum-norm solve when the Cholesky reports rank deficiency.
+ * V is taken from the plan's current Grams.  used_fallback (nullable) is set to 1
+ * when the QR path ran. */
+int itcpd_solve_ls(itcpd_plan* plan, int32_t mode, const double* M, double* X_out,
+                   int32_t mem, int32_t* used_fallback);
+
+/* row_norm (math_tools/row_norm.jl:4-24) followed by post_solve (algorithms/
+ * als_algorithms/standard/tensor.jl:46-49): A_n = X ./ lambda, lambda, G_n = A_n^T A_n,
+ * stored into the plan's CPD. */
+int itcpd_row_norm_post_solve(itcpd_plan* plan, int32_t mode, const double* X, int32_t mem);
+
+/* The FitCheck quantities (converge_checks/fit_check.jl:24-32, converge_checks.jl:5-11)
+ * for the plan's current CPD, with M = the MTTKRP of the last mode (host or device,
+ * I_N x R): inner = <T, X>, fact_square = ||X||^2, fit. */
+int itcpd_fit(itcpd_plan* plan, const double* M_last, int32_t mem, double ref_norm,
+              double* inner, double* fact_square, double* fit);
+
+/* norm(T) on the device (the ref_norm handed to FitCheck). */
+int itcpd_tensor_norm(itcpd_plan* plan, double* norm_out);
+
+/* ---- instrumentation ---- */
+int itcpd_plan_set_profiling(itcpd_plan* plan, int32_t on);
+int itcpd_get_kernel_stats(itcpd_plan* plan, itcpd_kernel_stats* stats);
+int itcpd_synchronize(itcpd_plan* plan);
+
+/* Thread-local text of the last failing call on this thread ("" if none). */
+const char* itcpd_last_error(void);
+
+/* Library version string. */
+const char* itcpd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ITCPD_H */

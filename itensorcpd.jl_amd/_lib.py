@@ -1,0 +1,98 @@
+"""ctypes binding of libitcpd_hip.so (the C ABI declared in include/itcpd.h)."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libitcpd_hip.so"
+
+ITCPD_OK = 0
+ITCPD_E_ARG = -1
+ITCPD_E_HIP = -2
+ITCPD_E_OOM = -3
+ITCPD_E_NAN = -4
+ITCPD_E_STATE = -5
+ITCPD_E_UNSUPPORTED = -6
+
+ITCPD_FP64, ITCPD_FP32, ITCPD_BF16X3 = 0, 1, 2
+ITCPD_ALG_KRP_FREE_NORMAL, ITCPD_ALG_KRP_NORMAL = 0, 1
+ITCPD_MEM_HOST, ITCPD_MEM_DEVICE = 0, 1
+ITCPD_CHECK_NONE, ITCPD_CHECK_FIT = 0, 1
+
+# Every function the header declares, with (restype, argtypes).
+_P = C.c_void_p
+_DP = C.POINTER(C.c_double)
+_I64P = C.POINTER(C.c_int64)
+
+
+class CheckStruct(C.Structure):
+    """Mirror of ``itcpd_check``."""
+    _fields_ = [
+        ("kind", C.c_int32), ("iter", C.c_int32), ("counter", C.c_int32),
+        ("max_counter", C.c_int32), ("total_iter", C.c_int32), ("sweeps_run", C.c_int32),
+        ("converged", C.c_int32), ("reserved", C.c_int32),
+        ("tolerance", C.c_double), ("ref_norm", C.c_double),
+        ("lastfit", C.c_double), ("final_fit", C.c_double),
+    ]
+
+
+class KernelStats(C.Structure):
+    """Mirror of ``itcpd_kernel_stats``."""
+    _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double),
+                ("flops", C.c_double), ("bytes", C.c_double)]
+
+
+SIGNATURES = {
+    "itcpd_plan_create": (C.c_int, [C.POINTER(_P), C.c_int32, _I64P, C.c_int64, C.c_int32, _P,
+                                    C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "itcpd_plan_destroy": (C.c_int, [_P]),
+    "itcpd_set_cpd": (C.c_int, [_P, C.POINTER(_P), _P, C.c_int32]),
+    "itcpd_get_cpd": (C.c_int, [_P, C.POINTER(_P), _P, C.c_int32]),
+    "itcpd_random_cpd": (C.c_int, [_P, C.c_uint64]),
+    "itcpd_random_factors": (C.c_int, [C.c_int32, _I64P, C.c_int64, C.c_uint64, C.POINTER(_P), _P,
+                                       C.c_int32, C.c_int32]),
+    "itcpd_run": (C.c_int, [_P, C.POINTER(CheckStruct), _DP]),
+    "itcpd_compute_krp": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
+    "itcpd_mttkrp": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
+    "itcpd_solve_ls": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int32, C.POINTER(C.c_int32)]),
+    "itcpd_row_norm_post_solve": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
+    "itcpd_fit": (C.c_int, [_P, _P, C.c_int32, C.c_double, _DP, _DP, _DP]),
+    "itcpd_tensor_norm": (C.c_int, [_P, _DP]),
+    "itcpd_plan_set_profiling": (C.c_int, [_P, C.c_int32]),
+    "itcpd_get_kernel_stats": (C.c_int, [_P, C.POINTER(KernelStats)]),
+    "itcpd_synchronize": (C.c_int, [_P]),
+    "itcpd_last_error": (C.c_char_p, []),
+    "itcpd_version": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+class ITCPDError(RuntimeError):
+    """A non-zero itcpd_status; ``code`` holds the status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[itcpd {code}] {msg}")
+        self.code = code
+
+
+def load() -> C.CDLL:
+    """Loads the in-tree library; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(the engine has no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != ITCPD_OK:
+        raise ITCPDError(rc, load().itcpd_last_error().decode(errors="replace"))

@@ -1,0 +1,434 @@
+"""Python mirror of ITensorCPD.jl's CP-ALS API, running on libitcpd_hip.so.
+
+Argument meaning and error behaviour follow the reference (file:line cited per
+function).  Tensors are numpy arrays (copied to the GPU once per plan) or torch
+CUDA tensors (borrowed); Julia's column-major layout is used on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+
+# ----------------------------------------------------------------- algorithms
+class CPDOptimizer:
+    """``abstract type CPDOptimizer`` (optimizers/cpd_optimizers.jl:1)."""
+
+
+class MttkrpAlgorithm:
+    """``abstract type MttkrpAlgorithm`` (algorithms/.../standard/MttkrpAlgorithm.jl:2)."""
+
+
+class KRPFreeNormal(MttkrpAlgorithm):
+    """algorithms/als_algorithms/standard/tensor.jl:30 (the default algorithm)."""
+
+
+class KRPNormal(MttkrpAlgorithm):
+    """algorithms/als_algorithms/standard/tensor.jl:10."""
+
+
+def _alg_code(alg) -> int:
+    if alg is None or isinstance(alg, KRPFreeNormal) or alg is KRPFreeNormal:
+        return L.ITCPD_ALG_KRP_FREE_NORMAL
+    if isinstance(alg, KRPNormal) or alg is KRPNormal:
+        return L.ITCPD_ALG_KRP_NORMAL
+    raise NotImplementedError(
+        f"algorithm {type(alg).__name__} is not on the GPU path (KRPFreeNormal / KRPNormal only)")
+
+
+# --------------------------------------------------------------------- checks
+class NoCheck:
+    """no_check.jl:1-6: a fixed number of sweeps."""
+
+    def __init__(self, max_counter: int):
+        self.iter = 0
+        self.max_counter = int(max_counter)
+        self.lastfit = -1.0
+
+
+class FitCheck:
+    """fit_check.jl:5-20: stop after two consecutive |delta fit| < tol."""
+
+    def __init__(self, tol: float, max_counter: int, norm: float):
+        self.iter = 0
+        self.counter = 0
+        self.tolerance = float(tol)
+        self.max_counter = int(max_counter)
+        self.ref_norm = float(norm)
+        self.MttKRP = None
+        self.lastfit = 1.0
+        self.final_fit = 0.0
+        self.total_iter = 0
+
+
+def CPDFit(check: FitCheck) -> float:
+    """fit_check.jl:68."""
+    return check.final_fit
+
+
+class ALSNaNError(RuntimeError):
+    """Raised where the reference does ``throw("Error NAN")`` (fit_check.jl:40-42)."""
+
+
+# ------------------------------------------------------------------------ CPD
+class CPD:
+    """cpd.jl:7-11: factor matrices (I_n x R, column-major float64) and lambda (R)."""
+
+    def __init__(self, factors: Sequence[np.ndarray], lam: np.ndarray):
+        self.factors: List[np.ndarray] = [np.asfortranarray(f, dtype=np.float64) for f in factors]
+        self.lam = np.ascontiguousarray(lam, dtype=np.float64)
+
+    def __getitem__(self, i):
+        return self.factors[i]
+
+    def __len__(self):
+        return len(self.factors)
+
+    @property
+    def rank(self) -> int:
+        return int(self.lam.shape[0])
+
+    def copy(self) -> "CPD":
+        """cpd.jl:33-35."""
+        return CPD([f.copy(order="F") for f in self.factors], self.lam.copy())
+
+    def isequal(self, other: "CPD") -> bool:
+        """cpd.jl:27-29."""
+        return all(np.array_equal(a, b) for a, b in zip(self.factors, other.factors)) and \
+            np.array_equal(self.lam, other.lam)
+
+    __eq__ = isequal
+
+
+def cp_rank(cp: CPD) -> int:
+    """cpd.jl:31."""
+    return cp.rank
+
+
+# ------------------------------------------------------------ tensor handling
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _colmajor_torch(t):
+    nd = t.dim()
+    if all(t.stride(k) == int(np.prod(t.shape[:k])) for k in range(nd)):
+        return t
+    rev = list(reversed(range(nd)))
+    return t.permute(*rev).contiguous().permute(*rev)
+
+
+class _Target:
+    """Resolves a tensor argument to (dims, elt_bytes, pointer, mem, keep-alive)."""
+
+    def __init__(self, A):
+        if _is_torch(A):
+            import torch
+            if A.is_cuda:
+                if A.dtype not in (torch.float64, torch.float32):
+                    A = A.to(torch.float64)
+                A = _colmajor_torch(A)
+                self.dims = tuple(int(d) for d in A.shape)
+                self.elt = A.element_size()
+                self.ptr = A.data_ptr()
+                self.mem = L.ITCPD_MEM_DEVICE
+                self.device = A.device.index or 0
+                self.keep = A
+                return
+            A = A.detach().numpy()
+        a = np.asarray(A)
+        if a.dtype not in (np.float64, np.float32):
+            a = a.astype(np.float64)
+        a = np.asfortranarray(a)
+        self.dims = tuple(int(d) for d in a.shape)
+        self.elt = a.itemsize
+        self.ptr = a.ctypes.data
+        self.mem = L.ITCPD_MEM_HOST
+        self.device = 0
+        self.keep = a
+
+
+def _ptr_array(ptrs):
+    arr = (C.c_void_p * len(ptrs))(*ptrs)
+    return arr
+
+
+def _seed_of(rng) -> int:
+    """rng=nothing -> the reference's fixed default stream (MersenneTwister(3), cpd.jl:50)."""
+    if rng is None:
+        return 3
+    if isinstance(rng, (int, np.integer)):
+        return int(rng) & ((1 << 64) - 1)
+    if hasattr(rng, "integers"):
+        return int(rng.integers(0, 2 ** 63))
+    raise TypeError("rng must be None, an int seed or a numpy Generator")
+
+
+# ------------------------------------------------------------------------ plan
+class ALSPlan:
+    """A device-resident ALS problem (the ``ALS`` struct, als_optimizer.jl:5-10).
+
+    Binds the target once; the CPD can be set, run and read back repeatedly.
+    """
+
+    def __init__(self, target, rank: int, alg=None, prec: int = L.ITCPD_FP64, device=None):
+        lib = L.load()
+        self._t = _Target(target)
+        self.dims = self._t.dims
+        self.rank = int(rank)
+        dev = self._t.device if device is None else int(device)
+        dims = (C.c_int64 * len(self.dims))(*self.dims)
+        h = C.c_void_p()
+        L.check(lib.itcpd_plan_create(C.byref(h), len(self.dims), dims, self.rank, self._t.elt,
+                                      C.c_void_p(self._t.ptr), self._t.mem, int(prec),
+                                      _alg_code(alg), dev))
+        self._h = h
+        self._lib = lib
+        if self._t.mem == L.ITCPD_MEM_HOST:
+            self._t.keep = None          # copied to the device; host copy no longer needed
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.itcpd_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- CPD in / out
+    def set_cpd(self, cp: CPD):
+        if cp.rank != self.rank or len(cp.factors) != len(self.dims):
+            raise ValueError("CPD does not match the plan's rank / order")
+        fs = []
+        for f, d in zip(cp.factors, self.dims):
+            if f.shape != (d, self.rank):
+                raise ValueError(f"factor shape {f.shape} != {(d, self.rank)}")
+            fs.append(np.asfortranarray(f, dtype=np.float64))
+        lam = np.ascontiguousarray(cp.lam, dtype=np.float64)
+        L.check(self._lib.itcpd_set_cpd(self._h, _ptr_array([f.ctypes.data for f in fs]),
+                                        C.c_void_p(lam.ctypes.data), L.ITCPD_MEM_HOST))
+
+    def set_cpd_device(self, factors, lam):
+        """Device factors (torch CUDA, float64, I_n x R column-major) and lambda."""
+        fs = [_colmajor_torch(f) for f in factors]
+        self._keep_dev = (fs, lam)
+        L.check(self._lib.itcpd_set_cpd(self._h, _ptr_array([f.data_ptr() for f in fs]),
+                                        C.c_void_p(lam.data_ptr()), L.ITCPD_MEM_DEVICE))
+
+    def random_cpd(self, rng=None):
+        L.check(self._lib.itcpd_random_cpd(self._h, _seed_of(rng)))
+
+    def get_cpd(self) -> CPD:
+        fs = [np.empty((d, self.rank), order="F") for d in self.dims]
+        lam = np.empty(self.rank)
+        L.check(self._lib.itcpd_get_cpd(self._h, _ptr_array([f.ctypes.data for f in fs]),
+                                        C.c_void_p(lam.ctypes.data), L.ITCPD_MEM_HOST))
+        return CPD(fs, lam)
+
+    # -- the sweep loop (optimize.jl:6-35)
+    def run(self, check, verbose: bool = False) -> dict:
+        cs = L.CheckStruct()
+        if isinstance(check, FitCheck):
+            cs.kind = L.ITCPD_CHECK_FIT
+            cs.counter = check.counter
+            cs.tolerance = check.tolerance
+            cs.ref_norm = check.ref_norm
+            cs.total_iter = check.total_iter
+            cs.final_fit = check.final_fit
+        elif isinstance(check, NoCheck):
+            cs.kind = L.ITCPD_CHECK_NONE
+        else:
+            raise NotImplementedError(f"check {type(check).__name__} is not on the GPU path")
+        cs.iter = check.iter
+        cs.max_counter = check.max_counter
+        cs.lastfit = check.lastfit
+        iter0, lastfit0 = check.iter, check.lastfit
+        nmax = max(check.max_counter - check.iter, 0)
+        trace = np.zeros(max(nmax, 1))
+        rc = self._lib.itcpd_run(self._h, C.byref(cs),
+                                 trace.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc not in (L.ITCPD_OK, L.ITCPD_E_NAN):
+            L.check(rc)
+        check.iter = cs.iter
+        check.lastfit = cs.lastfit
+        if isinstance(check, FitCheck):
+            check.counter = cs.counter
+            check.total_iter = cs.total_iter
+            check.final_fit = cs.final_fit
+        fits = trace[:cs.sweeps_run].copy() if isinstance(check, FitCheck) else np.zeros(0)
+        if verbose:
+            _print_trace(check, self.rank, iter0, lastfit0, fits, cs.sweeps_run)
+        if rc == L.ITCPD_E_NAN:
+            raise ALSNaNError("Error NAN")
+        return {"sweeps": int(cs.sweeps_run), "converged": bool(cs.converged), "fits": fits}
+
+    # -- per-hook entry points
+    def compute_krp(self, mode: int) -> np.ndarray:
+        V = np.empty((self.rank, self.rank), order="F")
+        L.check(self._lib.itcpd_compute_krp(self._h, mode, C.c_void_p(V.ctypes.data),
+                                            L.ITCPD_MEM_HOST))
+        return V
+
+    def mttkrp(self, mode: int) -> np.ndarray:
+        M = np.empty((self.dims[mode], self.rank), order="F")
+        L.check(self._lib.itcpd_mttkrp(self._h, mode, C.c_void_p(M.ctypes.data), L.ITCPD_MEM_HOST))
+        return M
+
+    def solve_ls(self, mode: int, M: np.ndarray):
+        M = np.asfortranarray(M, dtype=np.float64)
+        X = np.empty_like(M, order="F")
+        fb = C.c_int32(0)
+        L.check(self._lib.itcpd_solve_ls(self._h, mode, C.c_void_p(M.ctypes.data),
+                                         C.c_void_p(X.ctypes.data), L.ITCPD_MEM_HOST,
+                                         C.byref(fb)))
+        return X, bool(fb.value)
+
+    def row_norm_post_solve(self, mode: int, X: np.ndarray):
+        X = np.asfortranarray(X, dtype=np.float64)
+        L.check(self._lib.itcpd_row_norm_post_solve(self._h, mode, C.c_void_p(X.ctypes.data),
+                                                    L.ITCPD_MEM_HOST))
+
+    def fit(self, M_last: np.ndarray, ref_norm: float):
+        M_last = np.asfortranarray(M_last, dtype=np.float64)
+        inner, fs, fit = C.c_double(), C.c_double(), C.c_double()
+        L.check(self._lib.itcpd_fit(self._h, C.c_void_p(M_last.ctypes.data), L.ITCPD_MEM_HOST,
+                                    float(ref_norm), C.byref(inner), C.byref(fs), C.byref(fit)))
+        return inner.value, fs.value, fit.value
+
+    def tensor_norm(self) -> float:
+        out = C.c_double()
+        L.check(self._lib.itcpd_tensor_norm(self._h, C.byref(out)))
+        return out.value
+
+    def set_profiling(self, on: bool):
+        L.check(self._lib.itcpd_plan_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_stats(self) -> dict:
+        st = L.KernelStats()
+        L.check(self._lib.itcpd_get_kernel_stats(self._h, C.byref(st)))
+        return {"launches": st.launches, "total_ms": st.total_ms, "flops": st.flops,
+                "bytes": st.bytes}
+
+    def synchronize(self):
+        L.check(self._lib.itcpd_synchronize(self._h))
+
+
+def _print_trace(check, R, iter0, lastfit0, fits, nsweeps):
+    """The verbose lines of fit_check.jl:36-38 / no_check.jl:12-14."""
+    if isinstance(check, FitCheck):
+        last = lastfit0
+        for k, f in enumerate(fits):
+            print(f"{R}\t {iter0 + k + 1} \t {f} \t {abs(last - f)}")
+            last = f
+    else:
+        for k in range(nsweeps):
+            print(f"{R}\t {iter0 + k + 1}")
+
+
+# ------------------------------------------------------------------ functions
+def random_CPD(target, rank: int, rng=None) -> CPD:
+    """cpd.jl:63-82: N(0,1) factors normalised by row_norm; lambda from the last mode.
+
+    The draws come from the device's Philox stream keyed by the seed (rng=None ->
+    seed 3, the reference's fixed default); ``target`` only supplies the dims.
+    """
+    lib = L.load()
+    if isinstance(target, (tuple, list)):
+        dims = tuple(int(d) for d in target)
+        device = 0
+    else:
+        dims = tuple(int(d) for d in target.shape)
+        device = (target.device.index or 0) if _is_torch(target) and target.is_cuda else 0
+    rank = int(rank)
+    fs = [np.empty((d, rank), order="F") for d in dims]
+    lam = np.empty(rank)
+    cd = (C.c_int64 * len(dims))(*dims)
+    L.check(lib.itcpd_random_factors(len(dims), cd, rank, _seed_of(rng),
+                                     _ptr_array([f.ctypes.data for f in fs]),
+                                     C.c_void_p(lam.ctypes.data), L.ITCPD_MEM_HOST, device))
+    return CPD(fs, lam)
+
+
+def als_optimize(target, cp: CPD, alg=None, check=None, maxiter=None, verbose=False) -> CPD:
+    """als_optimizer.jl:15-25 with the defaults of compute_als (:37-55):
+    alg = KRPFreeNormal(), check = NoCheck(maxiter or 100).  Returns a new CPD."""
+    check = NoCheck(100 if maxiter is None else maxiter) if check is None else check
+    with ALSPlan(target, cp.rank, alg=alg) as plan:
+        plan.set_cpd(cp)
+        plan.run(check, verbose=verbose)
+        return plan.get_cpd()
+
+
+def tensor_norm(target) -> float:
+    """norm(T) evaluated on the device."""
+    with ALSPlan(target, 1) as plan:
+        return plan.tensor_norm()
+
+
+def increase_cpd_rank(cpd: CPD, new_rank: int, rng=None) -> CPD:
+    """decompose.jl:71-83: fresh random factors at the new rank with the old columns
+    copied into the leading columns; lambda from the fresh draw."""
+    old = cpd.rank
+    if new_rank < old:
+        raise ValueError("new rank must be >= the current rank")
+    fresh = random_CPD(tuple(f.shape[0] for f in cpd.factors), new_rank, rng)
+    for o, f in zip(cpd.factors, fresh.factors):
+        f[:, :old] = o
+    return CPD(fresh.factors, fresh.lam)
+
+
+def decompose_adaptive(A, epsilon, max_rank: int, rng=None, alg=None, check=None, maxiter=None,
+                       verbose=False, start_rank: int = 1, rank_step: int = 1) -> CPD:
+    """decompose.jl:32-69: grow the rank by rank_step until 1 - CPDFit < epsilon."""
+    if verbose:
+        print(f"Starting with rank: {start_rank}")
+    current = start_rank
+    cpd = random_CPD(A, start_rank, rng)
+    check = FitCheck(1e-3, 100, tensor_norm(A)) if check is None else check
+    while True:
+        cpd = als_optimize(A, cpd, alg=alg, check=check, maxiter=maxiter, verbose=verbose)
+        check.iter = 0
+        if 1.0 - CPDFit(check) < epsilon:
+            return cpd
+        current += rank_step
+        if verbose:
+            print(f"\nIncreasing rank to: {current}")
+        if current > max_rank:
+            print(f"Optimization Failed to converge within rank {max_rank}")
+            return cpd
+        cpd = increase_cpd_rank(cpd, current, rng)
+
+
+def decompose(A, *args, solver=None, rng=None, alg=None, check=None, maxiter=None,
+              verbose=False, **kw) -> CPD:
+    """decompose.jl:1-30 (fixed rank) and :32-69 (decompose(A, epsilon, max_rank)).
+
+    ``solver`` must be None: a CPDOptimizer raises RuntimeError("OptimizerError")
+    (decompose.jl:28); any other type raises TypeError, as the reference's typed
+    keyword ``solver::Union{CPDOptimizer,Nothing}`` does (test/cp_als.jl:15).
+    """
+    if solver is not None and not isinstance(solver, CPDOptimizer):
+        raise TypeError("solver must be a CPDOptimizer or None")
+    if len(args) == 2:
+        return decompose_adaptive(A, args[0], args[1], rng=rng, alg=alg, check=check,
+                                  maxiter=maxiter, verbose=verbose, **kw)
+    if len(args) != 1:
+        raise TypeError("decompose(A, rank) or decompose(A, epsilon, max_rank)")
+    rank = int(args[0])
+    cp = random_CPD(A, rank, rng)
+    if solver is None:
+        return als_optimize(A, cp, alg=alg, check=check, maxiter=maxiter, verbose=verbose)
+    raise RuntimeError("OptimizerError")

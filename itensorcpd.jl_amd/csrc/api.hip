@@ -1,0 +1,561 @@
+// C ABI of libitcpd_hip.so (declared in include/itcpd.h).
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "internal.h"
+
+namespace itcpd {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+    g_last_error = std::string("HIP error '") + hipGetErrorString(e) + "' in " + what;
+    return e == hipErrorOutOfMemory ? ITCPD_E_OOM : ITCPD_E_HIP;
+}
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+template <typename T>
+int dmalloc(T** ptr, size_t count) {
+    *ptr = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)ptr, count * sizeof(T));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    return ITCPD_OK;
+}
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int choose_rb(int64_t R) {
+    if (R <= 16) return 1;
+    if (R <= 32) return 2;
+    if (R <= 64) return 4;
+    return 8;
+}
+
+// Dimension-tree split: minimise ML + MR (the partial sizes); ties -> larger s.
+int choose_split(int N, const int64_t* dims) {
+    int best = 1;
+    double bestc = -1;
+    for (int s = 1; s < N; ++s) {
+        double ml = 1, mr = 1;
+        for (int q = 0; q < s; ++q) ml *= (double)dims[q];
+        for (int q = s; q < N; ++q) mr *= (double)dims[q];
+        const double c = ml + mr;
+        if (bestc < 0 || c <= bestc) {
+            bestc = c;
+            best = s;
+        }
+    }
+    return best;
+}
+
+int copy_in(void* dst, const void* src, size_t bytes, int mem, hipStream_t st) {
+    ITCPD_HIP(hipMemcpyAsync(dst, src, bytes, mem == ITCPD_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                                      : hipMemcpyHostToDevice,
+                             st));
+    return ITCPD_OK;
+}
+int copy_out(void* dst, const void* src, size_t bytes, int mem, hipStream_t st) {
+    ITCPD_HIP(hipMemcpyAsync(dst, src, bytes, mem == ITCPD_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                                      : hipMemcpyDeviceToHost,
+                             st));
+    return ITCPD_OK;
+}
+
+// Copies an I x R column-major block into an I x Rp column-major device array.
+int put_factor(itcpd_plan* p, int m, const double* src, int mem) {
+    const int64_t In = p->dims[m];
+    return copy_in(p->A[m], src, (size_t)In * p->R * 8, mem, p->stream);
+}
+
+int reset_check_flags(itcpd_plan* p) {
+    // clear done / nan so that hook calls after a converged run execute
+    DevCheck* h = p->chk_host;
+    ITCPD_HIP(hipMemcpyAsync(h, p->chk, sizeof(DevCheck), hipMemcpyDeviceToHost, p->stream));
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    h->done = 0;
+    h->nan_flag = 0;
+    ITCPD_HIP(hipMemcpyAsync(p->chk, h, sizeof(DevCheck), hipMemcpyHostToDevice, p->stream));
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    return ITCPD_OK;
+}
+
+int collect_timer(itcpd_plan* p) {
+    KernelTimer& tm = p->timer;
+    const size_t npair = tm.ev.size() / 2;
+    for (size_t k = 0; k < npair; ++k) {
+        float ms = 0.f;
+        ITCPD_HIP(hipEventSynchronize(tm.ev[2 * k + 1]));
+        ITCPD_HIP(hipEventElapsedTime(&ms, tm.ev[2 * k], tm.ev[2 * k + 1]));
+        tm.total_ms += ms;
+        tm.flops += tm.pending_flops[k];
+        tm.bytes += tm.pending_bytes[k];
+        tm.launches += 1;
+    }
+    for (hipEvent_t e : tm.ev) (void)hipEventDestroy(e);
+    tm.ev.clear();
+    tm.pending_flops.clear();
+    tm.pending_bytes.clear();
+    return ITCPD_OK;
+}
+
+// Per-mode factor state shared by plans and the plan-free random init.
+int alloc_factor_state(itcpd_plan* p) {
+    int rc;
+    const int64_t R = p->R, Rp = p->Rp, maxI = p->maxI;
+    for (int q = 0; q < p->N; ++q) {
+        if ((rc = dmalloc(&p->A[q], (size_t)(p->dims[q] * Rp)))) return rc;
+        if ((rc = dmalloc(&p->At[q], (size_t)(p->dims[q] * Rp)))) return rc;
+        if ((rc = dmalloc(&p->G[q], (size_t)(R * R)))) return rc;
+        ITCPD_HIP(hipMemset(p->A[q], 0, (size_t)(p->dims[q] * Rp) * 8));
+        ITCPD_HIP(hipMemset(p->At[q], 0, (size_t)(p->dims[q] * Rp) * 8));
+    }
+    if ((rc = dmalloc(&p->lam, (size_t)Rp))) return rc;
+    ITCPD_HIP(hipMemset(p->lam, 0, (size_t)Rp * 8));
+    if ((rc = dmalloc(&p->X, (size_t)(maxI * Rp)))) return rc;
+    if ((rc = dmalloc(&p->M, (size_t)(maxI * Rp)))) return rc;
+    if ((rc = dmalloc(&p->sumsq_part, (size_t)(128 * R)))) return rc;
+    {
+        // Gram partials: at most 64 blocks and at most 2^25 doubles
+        int64_t nb = std::min<int64_t>(64, std::max<int64_t>(1, ((int64_t)1 << 25) / (R * R)));
+        if ((rc = dmalloc(&p->gram_part, (size_t)(nb * R * R)))) return rc;
+        if ((rc = dmalloc(&p->inner_part, (size_t)nb))) return rc;
+    }
+    if ((rc = dmalloc(&p->scal, 8))) return rc;
+    return ITCPD_OK;
+}
+
+int sweep(itcpd_plan* p) {
+    int rc;
+    const int N = p->N;
+    if ((rc = launch_left_gemm(p))) return rc;
+    for (int n = 0; n < p->s; ++n) {
+        if (p->s > 1 && (rc = launch_side_ttv(p, n))) return rc;
+        if ((rc = launch_update_mode(p, n, n == N - 1))) return rc;
+    }
+    if ((rc = launch_right_gemm(p))) return rc;
+    for (int n = p->s; n < N; ++n) {
+        if (N - p->s > 1 && (rc = launch_side_ttv(p, n))) return rc;
+        if ((rc = launch_update_mode(p, n, n == N - 1))) return rc;
+    }
+    if (p->profiling && p->timer.ev.size() >= 64) return collect_timer(p);
+    return ITCPD_OK;
+}
+
+}  // namespace
+}  // namespace itcpd
+
+using namespace itcpd;
+
+extern "C" {
+
+const char* itcpd_last_error(void) { return g_last_error.c_str(); }
+
+const char* itcpd_version(void) { return "itcpd-hip 0.1 (gfx950)"; }
+
+int itcpd_plan_create(itcpd_plan** out, int32_t order, const int64_t* dims, int64_t rank,
+                      int32_t elt_bytes, const void* tensor, int32_t tensor_mem, int32_t prec,
+                      int32_t alg, int32_t device) {
+    if (!out) return fail(ITCPD_E_ARG, "plan pointer is null");
+    *out = nullptr;
+    if (order < 2 || order > ITCPD_MAX_ORDER)
+        return fail(ITCPD_E_ARG, "tensor order must be between 2 and 8");
+    if (!dims || !tensor) return fail(ITCPD_E_ARG, "dims or tensor pointer is null");
+    if (rank < 1 || rank > ITCPD_MAX_RANK) return fail(ITCPD_E_ARG, "rank must be in [1, 1024]");
+    if (elt_bytes != 8 && elt_bytes != 4) return fail(ITCPD_E_ARG, "elt_bytes must be 8 or 4");
+    if (tensor_mem != ITCPD_MEM_HOST && tensor_mem != ITCPD_MEM_DEVICE)
+        return fail(ITCPD_E_ARG, "tensor_mem must be ITCPD_MEM_HOST or ITCPD_MEM_DEVICE");
+    if (prec != ITCPD_FP64)
+        return fail(ITCPD_E_UNSUPPORTED, "only the FP64 MTTKRP is available in this build");
+    if (alg != ITCPD_ALG_KRP_FREE_NORMAL && alg != ITCPD_ALG_KRP_NORMAL)
+        return fail(ITCPD_E_ARG, "alg must be KRPFreeNormal or KRPNormal");
+    for (int q = 0; q < order; ++q)
+        if (dims[q] < 1) return fail(ITCPD_E_ARG, "every dimension must be >= 1");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(ITCPD_E_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(ITCPD_E_ARG, "device index out of range");
+
+    itcpd_plan* p = new (std::nothrow) itcpd_plan();
+    if (!p) return fail(ITCPD_E_OOM, "host allocation failed");
+    DeviceGuard dg(device);
+    p->device = device;
+    p->N = order;
+    for (int q = 0; q < order; ++q) p->dims[q] = dims[q];
+    p->R = rank;
+    p->RB = choose_rb(rank);
+    p->Rp = cdiv(rank, 16 * p->RB) * 16 * p->RB;
+    p->elt_bytes = elt_bytes;
+    p->prec = prec;
+    p->alg = alg;
+    p->s = choose_split(order, dims);
+    p->ML = 1;
+    p->MR = 1;
+    for (int q = 0; q < p->s; ++q) p->ML *= dims[q];
+    for (int q = p->s; q < order; ++q) p->MR *= dims[q];
+    p->maxI = 0;
+    for (int q = 0; q < order; ++q) p->maxI = dims[q] > p->maxI ? dims[q] : p->maxI;
+
+    int rc = ITCPD_OK;
+    auto bail = [&](int code) {
+        itcpd_plan_destroy(p);
+        return code;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+
+    const int64_t R = p->R, Rp = p->Rp, maxI = p->maxI;
+    const size_t nel = (size_t)(p->ML * p->MR);
+    if (tensor_mem == ITCPD_MEM_HOST) {
+        if ((rc = dmalloc((char**)&p->T_owned, nel * elt_bytes))) return bail(rc);
+        e = hipMemcpy(p->T_owned, tensor, nel * elt_bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy(tensor)"));
+        p->T = p->T_owned;
+    } else {
+        e = hipDeviceSynchronize();   // the tensor may have been produced on another stream
+        if (e != hipSuccess) return bail(hip_fail(e, "hipDeviceSynchronize"));
+        if (((uintptr_t)tensor) % 16 != 0) return bail(fail(ITCPD_E_ARG, "device tensor must be 16-byte aligned"));
+        p->T = tensor;
+    }
+    if ((rc = alloc_factor_state(p))) return bail(rc);
+    if ((rc = dmalloc(&p->U, (size_t)(R * R)))) return bail(rc);
+    if ((rc = dmalloc(&p->qr_aux, (size_t)(R * R + 4 * R)))) return bail(rc);
+    if ((rc = dmalloc(&p->piv, (size_t)(2 * R)))) return bail(rc);
+    if ((rc = dmalloc(&p->finfo, 4))) return bail(rc);
+    if (p->s > 1 && (rc = dmalloc(&p->PL, (size_t)(p->ML * R)))) return bail(rc);
+    if (order - p->s > 1 && (rc = dmalloc(&p->PR, (size_t)(p->MR * R)))) return bail(rc);
+    {
+        const size_t outmax = (size_t)std::max<int64_t>(std::max(p->ML, p->MR), maxI) * R;
+        size_t want = std::min<size_t>((size_t)1 << 24, outmax * 512);
+        want = std::max<size_t>(want, 1024 * 4);
+        p->parts_elems = want;
+        if ((rc = dmalloc(&p->parts, want))) return bail(rc);
+    }
+    if ((rc = dmalloc(&p->chk, 1))) return bail(rc);
+    e = hipHostMalloc((void**)&p->chk_host, sizeof(DevCheck), hipHostMallocDefault);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipHostMalloc"));
+    std::memset(p->chk_host, 0, sizeof(DevCheck));
+    e = hipMemcpy(p->chk, p->chk_host, sizeof(DevCheck), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy(check)"));
+    p->trace_cap = 1024;
+    if ((rc = dmalloc(&p->fit_trace, (size_t)p->trace_cap))) return bail(rc);
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) return bail(hip_fail(e, "hipDeviceSynchronize"));
+    *out = p;
+    return ITCPD_OK;
+}
+
+int itcpd_plan_destroy(itcpd_plan* p) {
+    if (!p) return ITCPD_OK;
+    DeviceGuard dg(p->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    for (hipEvent_t e : p->timer.ev) (void)hipEventDestroy(e);
+    dfree(p->T_owned);
+    for (int q = 0; q < ITCPD_MAX_ORDER; ++q) {
+        dfree(p->A[q]);
+        dfree(p->At[q]);
+        dfree(p->G[q]);
+    }
+    dfree(p->lam);
+    dfree(p->X);
+    dfree(p->M);
+    dfree(p->U);
+    dfree(p->qr_aux);
+    dfree(p->piv);
+    dfree(p->finfo);
+    dfree(p->PL);
+    dfree(p->PR);
+    dfree(p->parts);
+    dfree(p->sumsq_part);
+    dfree(p->gram_part);
+    dfree(p->inner_part);
+    dfree(p->scal);
+    dfree(p->chk);
+    dfree(p->fit_trace);
+    if (p->chk_host) (void)hipHostFree(p->chk_host);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return ITCPD_OK;
+}
+
+int itcpd_set_cpd(itcpd_plan* p, const double* const* factors, const double* lambda, int32_t mem) {
+    if (!p || !factors) return fail(ITCPD_E_ARG, "null argument");
+    DeviceGuard dg(p->device);
+    int rc;
+    for (int m = 0; m < p->N; ++m) {
+        if (!factors[m]) return fail(ITCPD_E_ARG, "null factor pointer");
+        if ((rc = put_factor(p, m, factors[m], mem))) return rc;
+    }
+    if (lambda && (rc = copy_in(p->lam, lambda, (size_t)p->R * 8, mem, p->stream))) return rc;
+    if ((rc = reset_check_flags(p))) return rc;
+    if ((rc = launch_seed_grams(p))) return rc;
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    p->have_cpd = true;
+    return ITCPD_OK;
+}
+
+int itcpd_random_cpd(itcpd_plan* p, uint64_t seed) {
+    if (!p) return fail(ITCPD_E_ARG, "null plan");
+    DeviceGuard dg(p->device);
+    int rc;
+    if ((rc = reset_check_flags(p))) return rc;
+    if ((rc = launch_random_init(p, seed))) return rc;
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    p->have_cpd = true;
+    return ITCPD_OK;
+}
+
+int itcpd_random_factors(int32_t order, const int64_t* dims, int64_t rank, uint64_t seed,
+                         double* const* factors, double* lambda, int32_t mem, int32_t device) {
+    if (order < 1 || order > ITCPD_MAX_ORDER) return fail(ITCPD_E_ARG, "order must be in [1, 8]");
+    if (!dims || !factors) return fail(ITCPD_E_ARG, "null argument");
+    if (rank < 1 || rank > ITCPD_MAX_RANK) return fail(ITCPD_E_ARG, "rank must be in [1, 1024]");
+    for (int q = 0; q < order; ++q)
+        if (dims[q] < 1 || !factors[q]) return fail(ITCPD_E_ARG, "bad dims or factor pointer");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(ITCPD_E_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(ITCPD_E_ARG, "device index out of range");
+    // a plan without a tensor: only the factor state is allocated
+    itcpd_plan* p = new (std::nothrow) itcpd_plan();
+    if (!p) return fail(ITCPD_E_OOM, "host allocation failed");
+    DeviceGuard dg(device);
+    auto bail = [&](int code) {
+        itcpd_plan_destroy(p);
+        return code;
+    };
+    p->device = device;
+    p->N = order;
+    p->R = rank;
+    p->RB = choose_rb(rank);
+    p->Rp = cdiv(rank, 16 * p->RB) * 16 * p->RB;
+    for (int q = 0; q < order; ++q) {
+        p->dims[q] = dims[q];
+        p->maxI = dims[q] > p->maxI ? dims[q] : p->maxI;
+    }
+    int rc;
+    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+    if ((rc = alloc_factor_state(p))) return bail(rc);
+    if ((rc = launch_random_init(p, seed))) return bail(rc);
+    p->have_cpd = true;
+    rc = itcpd_get_cpd(p, factors, lambda, mem);
+    return bail(rc);
+}
+
+int itcpd_get_cpd(itcpd_plan* p, double* const* factors, double* lambda, int32_t mem) {
+    if (!p) return fail(ITCPD_E_ARG, "null plan");
+    if (!p->have_cpd) return fail(ITCPD_E_STATE, "no CPD set on this plan");
+    DeviceGuard dg(p->device);
+    int rc;
+    for (int m = 0; factors && m < p->N; ++m) {
+        if (!factors[m]) continue;
+        if ((rc = copy_out(factors[m], p->A[m], (size_t)p->dims[m] * p->R * 8, mem, p->stream)))
+            return rc;
+    }
+    if (lambda && (rc = copy_out(lambda, p->lam, (size_t)p->R * 8, mem, p->stream))) return rc;
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    return ITCPD_OK;
+}
+
+int itcpd_run(itcpd_plan* p, itcpd_check* c, double* fit_trace) {
+    if (!p || !c) return fail(ITCPD_E_ARG, "null argument");
+    if (!p->have_cpd) return fail(ITCPD_E_STATE, "itcpd_run before itcpd_set_cpd / itcpd_random_cpd");
+    if (c->kind != ITCPD_CHECK_NONE && c->kind != ITCPD_CHECK_FIT)
+        return fail(ITCPD_E_ARG, "unknown check kind");
+    DeviceGuard dg(p->device);
+    // optimize.jl:8: the sweep loop runs while the local counter (starting at
+    // check.iter) is below check.max_counter
+    const int64_t nsweeps = (int64_t)c->max_counter - c->iter;
+    c->sweeps_run = 0;
+    c->converged = 0;
+    if (nsweeps <= 0) return ITCPD_OK;
+    if (nsweeps > p->trace_cap) {
+        dfree(p->fit_trace);
+        p->fit_trace = nullptr;
+        p->trace_cap = (int)nsweeps;
+        int rc = dmalloc(&p->fit_trace, (size_t)p->trace_cap);
+        if (rc) return rc;
+    }
+    DevCheck* h = p->chk_host;
+    std::memset(h, 0, sizeof(DevCheck));
+    h->kind = c->kind;
+    h->iter = c->iter;
+    h->counter = c->counter;
+    h->max_counter = c->max_counter;
+    h->total_iter = c->total_iter;
+    h->tolerance = c->tolerance;
+    h->ref_norm = c->ref_norm;
+    h->lastfit = c->lastfit;
+    h->final_fit = c->final_fit;
+    ITCPD_HIP(hipMemcpyAsync(p->chk, h, sizeof(DevCheck), hipMemcpyHostToDevice, p->stream));
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+
+    const int64_t batch = (c->kind == ITCPD_CHECK_FIT) ? 4 : nsweeps;
+    int rc = ITCPD_OK;
+    for (int64_t k = 0; k < nsweeps;) {
+        const int64_t nb = std::min<int64_t>(batch, nsweeps - k);
+        for (int64_t b = 0; b < nb; ++b)
+            if ((rc = sweep(p))) return rc;
+        k += nb;
+        if (c->kind == ITCPD_CHECK_FIT && k < nsweeps) {
+            ITCPD_HIP(hipMemcpyAsync(h, p->chk, sizeof(DevCheck), hipMemcpyDeviceToHost, p->stream));
+            ITCPD_HIP(hipStreamSynchronize(p->stream));
+            if (h->done) break;
+        }
+    }
+    ITCPD_HIP(hipMemcpyAsync(h, p->chk, sizeof(DevCheck), hipMemcpyDeviceToHost, p->stream));
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    if (p->profiling && (rc = collect_timer(p))) return rc;
+    c->iter = h->iter;
+    c->counter = h->counter;
+    c->total_iter = h->total_iter;
+    c->lastfit = h->lastfit;
+    c->final_fit = h->final_fit;
+    c->sweeps_run = h->sweeps > 0 ? h->sweeps : (int)std::min<int64_t>(nsweeps, INT32_MAX);
+    c->converged = (h->done && !h->nan_flag) ? 1 : 0;
+    if (c->kind == ITCPD_CHECK_NONE) c->sweeps_run = h->sweeps;
+    if (fit_trace && c->kind == ITCPD_CHECK_FIT && h->sweeps > 0)
+        ITCPD_HIP(hipMemcpy(fit_trace, p->fit_trace, (size_t)std::min(h->sweeps, p->trace_cap) * 8,
+                            hipMemcpyDeviceToHost));
+    if (h->nan_flag) return fail(ITCPD_E_NAN, "Error NAN");
+    return ITCPD_OK;
+}
+
+int itcpd_compute_krp(itcpd_plan* p, int32_t mode, double* V_out, int32_t mem) {
+    if (!p || !V_out) return fail(ITCPD_E_ARG, "null argument");
+    if (mode < 0 || mode >= p->N) return fail(ITCPD_E_ARG, "mode out of range");
+    if (!p->have_cpd) return fail(ITCPD_E_STATE, "no CPD set");
+    DeviceGuard dg(p->device);
+    int rc;
+    if ((rc = launch_hadamard(p, mode, p->U))) return rc;
+    if ((rc = copy_out(V_out, p->U, (size_t)(p->R * p->R) * 8, mem, p->stream))) return rc;
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    return ITCPD_OK;
+}
+
+int itcpd_mttkrp(itcpd_plan* p, int32_t mode, double* M_out, int32_t mem) {
+    if (!p || !M_out) return fail(ITCPD_E_ARG, "null argument");
+    if (mode < 0 || mode >= p->N) return fail(ITCPD_E_ARG, "mode out of range");
+    if (!p->have_cpd) return fail(ITCPD_E_STATE, "no CPD set");
+    DeviceGuard dg(p->device);
+    int rc;
+    if ((rc = reset_check_flags(p))) return rc;
+    if ((rc = mttkrp_mode(p, mode))) return rc;
+    if ((rc = copy_out(M_out, p->M, (size_t)p->dims[mode] * p->R * 8, mem, p->stream))) return rc;
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    return ITCPD_OK;
+}
+
+int itcpd_solve_ls(itcpd_plan* p, int32_t mode, const double* M, double* X_out, int32_t mem,
+                   int32_t* used_fallback) {
+    if (!p || !M || !X_out) return fail(ITCPD_E_ARG, "null argument");
+    if (mode < 0 || mode >= p->N) return fail(ITCPD_E_ARG, "mode out of range");
+    if (!p->have_cpd) return fail(ITCPD_E_STATE, "no CPD set");
+    DeviceGuard dg(p->device);
+    int rc;
+    if ((rc = reset_check_flags(p))) return rc;
+    const size_t bytes = (size_t)p->dims[mode] * p->R * 8;
+    if ((rc = copy_in(p->M, M, bytes, mem, p->stream))) return rc;
+    if ((rc = launch_factor(p, mode))) return rc;
+    if ((rc = launch_solve_rows(p, mode, p->M))) return rc;
+    if ((rc = copy_out(X_out, p->X, bytes, mem, p->stream))) return rc;
+    int info[4];
+    ITCPD_HIP(hipMemcpyAsync(info, p->finfo, sizeof(info), hipMemcpyDeviceToHost, p->stream));
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    if (used_fallback) *used_fallback = info[1];
+    return ITCPD_OK;
+}
+
+int itcpd_row_norm_post_solve(itcpd_plan* p, int32_t mode, const double* X, int32_t mem) {
+    if (!p || !X) return fail(ITCPD_E_ARG, "null argument");
+    if (mode < 0 || mode >= p->N) return fail(ITCPD_E_ARG, "mode out of range");
+    if (!p->have_cpd) return fail(ITCPD_E_STATE, "no CPD set");
+    DeviceGuard dg(p->device);
+    int rc;
+    if ((rc = reset_check_flags(p))) return rc;
+    const int64_t In = p->dims[mode];
+    if ((rc = copy_in(p->X, X, (size_t)In * p->R * 8, mem, p->stream))) return rc;
+    if ((rc = launch_col_sumsq(p, mode, p->X))) return rc;
+    if ((rc = launch_normalize(p, mode, p->X, false))) return rc;
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    return ITCPD_OK;
+}
+
+int itcpd_fit(itcpd_plan* p, const double* M_last, int32_t mem, double ref_norm, double* inner,
+              double* fact_square, double* fit) {
+    if (!p || !M_last) return fail(ITCPD_E_ARG, "null argument");
+    if (!p->have_cpd) return fail(ITCPD_E_STATE, "no CPD set");
+    DeviceGuard dg(p->device);
+    int rc;
+    const int N = p->N;
+    if ((rc = copy_in(p->M, M_last, (size_t)p->dims[N - 1] * p->R * 8, mem, p->stream))) return rc;
+    if ((rc = launch_fit_only(p, p->M, ref_norm))) return rc;
+    double sc[3];
+    ITCPD_HIP(hipMemcpyAsync(sc, p->scal, sizeof(sc), hipMemcpyDeviceToHost, p->stream));
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    if (inner) *inner = sc[0];
+    if (fact_square) *fact_square = sc[1];
+    if (fit) *fit = sc[2];
+    return ITCPD_OK;
+}
+
+int itcpd_tensor_norm(itcpd_plan* p, double* norm_out) {
+    if (!p || !norm_out) return fail(ITCPD_E_ARG, "null argument");
+    DeviceGuard dg(p->device);
+    int rc;
+    if ((rc = launch_tensor_norm(p))) return rc;
+    double ss = 0.0;
+    ITCPD_HIP(hipMemcpyAsync(&ss, p->scal + 3, 8, hipMemcpyDeviceToHost, p->stream));
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    *norm_out = std::sqrt(ss);
+    return ITCPD_OK;
+}
+
+int itcpd_plan_set_profiling(itcpd_plan* p, int32_t on) {
+    if (!p) return fail(ITCPD_E_ARG, "null plan");
+    p->profiling = on != 0;
+    p->timer = KernelTimer();
+    return ITCPD_OK;
+}
+
+int itcpd_get_kernel_stats(itcpd_plan* p, itcpd_kernel_stats* st) {
+    if (!p || !st) return fail(ITCPD_E_ARG, "null argument");
+    DeviceGuard dg(p->device);
+    int rc;
+    if ((rc = collect_timer(p))) return rc;
+    st->launches = p->timer.launches;
+    st->total_ms = p->timer.total_ms;
+    st->flops = p->timer.flops;
+    st->bytes = p->timer.bytes;
+    return ITCPD_OK;
+}
+
+int itcpd_synchronize(itcpd_plan* p) {
+    if (!p) return fail(ITCPD_E_ARG, "null plan");
+    DeviceGuard dg(p->device);
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    return ITCPD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "itcpd.h"
+
+#define ITCPD_MAX_ORDER 8
+#define ITCPD_MAX_RANK 1024
+
+namespace itcpd {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f64_16x16x4f64 on gfx950 (layout measured on the device):
+//   A operand: lane l holds A[l & 15][l >> 4]
+//   B operand: lane l holds B[l >> 4][l & 15]
+//   result   : lane l, register v holds D[(l >> 4) + 4 v][l & 15]
+__device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Device-side copy of the convergence-check state (FitCheck / NoCheck fields).
+struct DevCheck {
+    int kind;
+    int iter;
+    int counter;
+    int max_counter;
+    int total_iter;
+    int done;        // check_converge returned true, or NaN: later sweeps are no-ops
+    int nan_flag;
+    int sweeps;      // sweeps executed since the last reset
+    double tolerance;
+    double ref_norm;
+    double lastfit;
+    double final_fit;
+};
+
+// One side of the dimension tree: the modes a GEMM contracts against T.
+struct Side {
+    int nm;                              // number of modes on this side
+    int first;                           // index of the first mode of the side
+    int64_t dim[ITCPD_MAX_ORDER];        // dims of the side's modes (first is fastest)
+    const double* fac[ITCPD_MAX_ORDER];  // factor pointers, layout set by the consumer
+};
+
+struct KernelTimer {
+    std::vector<hipEvent_t> ev;          // begin/end pairs recorded this run
+    int64_t launches = 0;
+    double total_ms = 0, flops = 0, bytes = 0;
+    std::vector<double> pending_flops, pending_bytes;
+};
+
+}  // namespace itcpd
+
+struct itcpd_plan {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int N = 0;
+    int64_t dims[ITCPD_MAX_ORDER] = {0};
+    int64_t R = 0, Rp = 0;   // rank and padded rank (multiple of 16 * RB)
+    int RB = 4;              // rank blocks of 16 per wave in the GEMMs
+    int elt_bytes = 8;
+    const void* T = nullptr;
+    void* T_owned = nullptr;
+    int prec = ITCPD_FP64;
+    int alg = ITCPD_ALG_KRP_FREE_NORMAL;
+    int s = 1;               // dimension-tree split: left = modes [0, s), right = [s, N)
+    int64_t ML = 1, MR = 1;  // products of left / right dims
+    int64_t maxI = 0;
+    bool have_cpd = false;
+
+    // device state
+    double* A[ITCPD_MAX_ORDER] = {nullptr};    // I_n x Rp column-major (zero beyond R)
+    double* At[ITCPD_MAX_ORDER] = {nullptr};   // Rp x I_n (rank fastest, zero beyond R)
+    double* G[ITCPD_MAX_ORDER] = {nullptr};    // R x R
+    double* lam = nullptr;                     // Rp
+    double* X = nullptr;                       // maxI x R column-major (unnormalised solution)
+    double* M = nullptr;                       // maxI x R column-major (summed MTTKRP)
+    double* U = nullptr;                       // R x R factor workspace (Cholesky / QR)
+    double* qr_aux = nullptr;                  // QR fallback workspace (tau, Z reflectors)
+    int* piv = nullptr;                        // R
+    int* finfo = nullptr;                      // [0] = rank, [1] = used fallback, [2] = QR rank
+    double* PL = nullptr;                      // ML x Rp partial of the left side (if s > 1)
+    double* PR = nullptr;                      // MR x Rp partial of the right side (if N-s > 1)
+    double* parts = nullptr;                   // split-K partial buffer
+    size_t parts_elems = 0;
+    double* sumsq_part = nullptr;              // [nb][Rp]
+    double* gram_part = nullptr;               // [nb][R*R]
+    double* inner_part = nullptr;              // [nb]
+    double* scal = nullptr;                    // [0]=inner [1]=fact_square [2]=fit [3]=tensor norm^2
+    itcpd::DevCheck* chk = nullptr;
+    double* fit_trace = nullptr;               // device, length trace_cap
+    int trace_cap = 0;
+    itcpd::DevCheck* chk_host = nullptr;       // pinned mirror
+    bool profiling = false;
+    itcpd::KernelTimer timer;
+};
+
+namespace itcpd {
+
+// error plumbing (api.hip)
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+// launchers (mttkrp.hip)
+int launch_left_gemm(itcpd_plan* p);      // PL (or M of mode 0 when s == 1)
+int launch_right_gemm(itcpd_plan* p);     // PR (or M of mode N-1 when N-s == 1)
+int launch_side_ttv(itcpd_plan* p, int mode);  // M of `mode` from PL / PR
+int mttkrp_mode(itcpd_plan* p, int mode);      // full M_mode into p->M from the current factors
+int launch_tensor_norm(itcpd_plan* p);
+
+// launchers (solve.hip)
+int launch_update_mode(itcpd_plan* p, int mode, bool fit_after);  // solve + row_norm + post_solve
+int launch_factor(itcpd_plan* p, int mode);
+int launch_solve_rows(itcpd_plan* p, int mode, const double* Msrc);
+int launch_normalize(itcpd_plan* p, int mode, const double* Xsrc, bool fit_after);
+int launch_seed_grams(itcpd_plan* p);
+int launch_random_init(itcpd_plan* p, uint64_t seed);
+int launch_fit_only(itcpd_plan* p, const double* Mlast, double ref_norm);
+int launch_hadamard(itcpd_plan* p, int mode, double* V);
+int launch_col_sumsq(itcpd_plan* p, int mode, const double* X);
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace itcpd
+
+#define ITCPD_HIP(call)                                              \
+    do {                                                             \
+        hipError_t e_ = (call);                                      \
+        if (e_ != hipSuccess) return itcpd::hip_fail(e_, #call);     \
+    } while (0)

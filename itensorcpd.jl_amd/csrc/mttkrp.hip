@@ -1,0 +1,634 @@
+// MTTKRP as a dimension tree on FP64 MFMA (gfx950).
+//
+// The reference computes each M_n = T_(n) (KRP of the other factors) separately,
+// one full contraction of T per rank slice (algebra/had_contract.jl:111-121).
+// Here the modes are split into a left side [0, s) and a right side [s, N):
+//   left  GEMM  P_L(mL, r) = sum_mR T(mL, mR) KRP_right(mR, r)   ("TK": output rows contiguous in T)
+//   right GEMM  P_R(mR, r) = sum_mL T(mL, mR) KRP_left(mL, r)    ("TR": reduction contiguous in T)
+// with T viewed as an ML x MR column-major matrix (mL = left multi-index, mode 0
+// fastest).  The KRP operand is generated on the fly from the factors.  A side
+// with one mode yields its M directly; a side with several modes gets each M_n
+// from its partial by a Hadamard-reduce against the side's other (current)
+// factors.  T is read twice per sweep.
+#include "internal.h"
+
+namespace itcpd {
+
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+template <typename TT>
+__device__ __forceinline__ double ld1(const TT* p) { return (double)p[0]; }
+
+// Loads MB consecutive elements (MB even; the run must be 16-byte aligned for doubles).
+template <typename TT, int MB>
+__device__ __forceinline__ void ld_run(const TT* __restrict__ p, double* out) {
+    if constexpr (sizeof(TT) == 8) {
+        const dv2* q = reinterpret_cast<const dv2*>(p);
+#pragma unroll
+        for (int k = 0; k < MB / 2; ++k) {
+            dv2 v = q[k];
+            out[2 * k] = v.x;
+            out[2 * k + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < MB; ++k) out[k] = (double)p[k];
+    }
+}
+
+// Weight of a "rest" multi-index for the side's modes 1..nm-1 at rank r.
+// Factor layout: rank-fastest (Rp x I) when rank_fast, else column-major (I x Rp).
+__device__ __forceinline__ double rest_weight(const Side& sd, int64_t rest, int r, int Rp,
+                                              bool rank_fast) {
+    double w = 1.0;
+    int64_t t = rest;
+    for (int q = 1; q < sd.nm; ++q) {
+        const int64_t d = t % sd.dim[q];
+        t /= sd.dim[q];
+        w *= rank_fast ? sd.fac[q][d * Rp + r] : sd.fac[q][d + sd.dim[q] * r];
+    }
+    return w;
+}
+
+// ---------------------------------------------------------------- TK GEMM
+// out(mL, r) = sum over reduction index mR = i + I0*rest of T(mL, mR) KRP(mR, r),
+// KRP(mR, r) = At0[i*Rp + r] * prod_{q>=1} Atq[digit_q*Rp + r].
+// Wave tile: 16*MB rows x 16*RB ranks.  Lane (g = l>>4, c = l&15) owns rows
+// m0 + MB*c .. + MB-1 (a contiguous run, loaded as vectors) at reduction index g of
+// each 4-step.  The reduction range [kk0, kk1) counts 4-steps over (rest, i).
+template <typename TT, int RB, int MB, int U, bool FAST>
+__global__ __launch_bounds__(256, 2) void k_tk(const TT* __restrict__ T, int64_t ML, Side sd,
+                                               int64_t nsteps_i, int64_t kk_per_split,
+                                               int64_t kk_total, int Rp, int R,
+                                               double* __restrict__ out, int64_t out_ld,
+                                               int64_t split_stride,
+                                               const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int64_t m0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * MB);
+    if (m0 >= ML) return;
+    const int r0 = blockIdx.y * (16 * RB);
+    const int64_t kk0 = (int64_t)blockIdx.z * kk_per_split;
+    const int64_t kk1 = kk0 + kk_per_split < kk_total ? kk0 + kk_per_split : kk_total;
+    const int64_t I0 = sd.dim[0];
+    const double* __restrict__ At0 = sd.fac[0];
+    const int64_t mrow = m0 + MB * c;
+
+    d4 acc[RB][MB];
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int b = 0; b < MB; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+
+    int64_t kk = kk0;
+    while (kk < kk1) {
+        const int64_t rest = kk / nsteps_i;
+        const int64_t s_lo = kk - rest * nsteps_i;
+        int64_t s_hi = nsteps_i;
+        if ((rest + 1) * nsteps_i > kk1) s_hi = kk1 - rest * nsteps_i;
+        double w[RB];
+#pragma unroll
+        for (int a = 0; a < RB; ++a) w[a] = rest_weight(sd, rest, r0 + 16 * a + c, Rp, true);
+        const TT* __restrict__ Tr = T + (I0 * rest) * ML + mrow;
+
+        // two register stages: loads of the next U steps are issued before the
+        // MFMAs of the current ones
+        double ta[2][U][MB], ka[2][U][RB];
+        auto load = [&](int buf, int64_t st) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t i = (st + u) * 4 + g;
+                const bool ok = (st + u < s_hi) && (FAST || i < I0);
+                if constexpr (FAST) {
+                    if (ok) {
+                        ld_run<TT, MB>(Tr + i * ML, ta[buf][u]);
+                    } else {
+#pragma unroll
+                        for (int b = 0; b < MB; ++b) ta[buf][u][b] = 0.0;
+                    }
+                } else {
+#pragma unroll
+                    for (int b = 0; b < MB; ++b)
+                        ta[buf][u][b] = (ok && mrow + b < ML) ? ld1(Tr + i * ML + b) : 0.0;
+                }
+#pragma unroll
+                for (int a = 0; a < RB; ++a)
+                    ka[buf][u][a] = ok ? At0[i * Rp + r0 + 16 * a + c] * w[a] : 0.0;
+            }
+        };
+        auto compute = [&](int buf) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int a = 0; a < RB; ++a)
+#pragma unroll
+                    for (int b = 0; b < MB; ++b)
+                        acc[a][b] = mfma_f64(ka[buf][u][a], ta[buf][u][b], acc[a][b]);
+        };
+        int64_t st = s_lo;
+        load(0, st);
+        while (true) {
+            const bool more1 = st + U < s_hi;
+            if (more1) load(1, st + U);
+            compute(0);
+            st += U;
+            if (!more1) break;
+            const bool more0 = st + U < s_hi;
+            if (more0) load(0, st + U);
+            compute(1);
+            st += U;
+            if (!more0) break;
+        }
+        kk = rest * nsteps_i + s_hi;
+    }
+
+    double* o = out + (int64_t)blockIdx.z * split_stride;
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + 16 * a + g + 4 * v;
+            if (r >= R) continue;
+#pragma unroll
+            for (int b = 0; b < MB; ++b) {
+                const int64_t row = mrow + b;
+                if (FAST || row < ML) o[row + out_ld * r] = acc[a][b][v];
+            }
+        }
+}
+
+// ---------------------------------------------------------------- TR GEMM
+// out(mR, r) = sum over mL = i + I0*rest of T(mL, mR) KRP(mL, r),
+// KRP(mL, r) = A0[i + I0*r] * prod_{q>=1} Aq[digit_q + Iq*r] (column-major factors).
+// Wave tile: 16*MB output rows x 16*RB ranks.  Over a group of 4*S reduction
+// indices, lane (g, c) owns the run i0 + S*g .. + S-1 (contiguous in T and in A0).
+template <typename TT, int RB, int MB, int S, bool FAST>
+__global__ __launch_bounds__(256, 2) void k_tr(const TT* __restrict__ T, int64_t ML, int64_t MR,
+                                               Side sd, int64_t ngroups_i, int64_t kk_per_split,
+                                               int64_t kk_total, int Rp, int R,
+                                               double* __restrict__ out, int64_t split_stride,
+                                               const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int64_t n0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * MB);
+    if (n0 >= MR) return;
+    const int r0 = blockIdx.y * (16 * RB);
+    const int64_t kk0 = (int64_t)blockIdx.z * kk_per_split;
+    const int64_t kk1 = kk0 + kk_per_split < kk_total ? kk0 + kk_per_split : kk_total;
+    const int64_t I0 = sd.dim[0];
+    const double* __restrict__ A0 = sd.fac[0];
+
+    int64_t col[MB];
+    bool colok[MB];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        col[b] = n0 + 16 * b + c;
+        colok[b] = FAST || col[b] < MR;
+        if (!colok[b]) col[b] = 0;
+    }
+    const double* __restrict__ Arow[RB];
+#pragma unroll
+    for (int a = 0; a < RB; ++a) Arow[a] = A0 + I0 * (int64_t)(r0 + 16 * a + c);
+
+    d4 acc[RB][MB];
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int b = 0; b < MB; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+
+    int64_t kk = kk0;
+    while (kk < kk1) {
+        const int64_t rest = kk / ngroups_i;
+        const int64_t s_lo = kk - rest * ngroups_i;
+        int64_t s_hi = ngroups_i;
+        if ((rest + 1) * ngroups_i > kk1) s_hi = kk1 - rest * ngroups_i;
+        double w[RB];
+#pragma unroll
+        for (int a = 0; a < RB; ++a) w[a] = rest_weight(sd, rest, r0 + 16 * a + c, Rp, false);
+        const TT* __restrict__ Tr = T + I0 * rest;
+
+        double tb[2][S][MB], ka[2][S][RB];
+        auto compute = [&](int buf) {
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int a = 0; a < RB; ++a)
+#pragma unroll
+                    for (int b = 0; b < MB; ++b)
+                        acc[a][b] = mfma_f64(ka[buf][s][a], tb[buf][s][b], acc[a][b]);
+        };
+        auto loads = [&](int buf, int64_t grp) {
+            const int64_t ib = grp * (4 * S) + S * g;
+            if constexpr (FAST) {
+#pragma unroll
+                for (int b = 0; b < MB; ++b) {
+                    double run[S];
+                    ld_run<TT, S>(Tr + ib + ML * col[b], run);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) tb[buf][s][b] = run[s];
+                }
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+#pragma unroll
+                    for (int a = 0; a < RB; ++a) ka[buf][s][a] = Arow[a][ib + s] * w[a];
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int64_t i = ib + s;
+                    const bool ok = i < I0;
+#pragma unroll
+                    for (int b = 0; b < MB; ++b)
+                        tb[buf][s][b] = (ok && colok[b]) ? ld1(Tr + i + ML * col[b]) : 0.0;
+#pragma unroll
+                    for (int a = 0; a < RB; ++a) ka[buf][s][a] = ok ? Arow[a][i] * w[a] : 0.0;
+                }
+            }
+        };
+        int64_t gp = s_lo;
+        loads(0, gp);
+        while (true) {
+            const bool more1 = gp + 1 < s_hi;
+            if (more1) loads(1, gp + 1);
+            compute(0);
+            gp += 1;
+            if (!more1) break;
+            const bool more0 = gp + 1 < s_hi;
+            if (more0) loads(0, gp + 1);
+            compute(1);
+            gp += 1;
+            if (!more0) break;
+        }
+        kk = rest * ngroups_i + s_hi;
+    }
+
+    double* o = out + (int64_t)blockIdx.z * split_stride;
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + 16 * a + g + 4 * v;
+            if (r >= R) continue;
+#pragma unroll
+            for (int b = 0; b < MB; ++b)
+                if (colok[b]) o[col[b] + MR * r] = acc[a][b][v];
+        }
+}
+
+// ------------------------------------------------------- Hadamard-reduce (ttv)
+// M(i, r) = sum_{a<A, b<B} P(a + A*i + A*In*b, r) * Wa(a, r) * Wb(b, r)
+// Wa = product over the side's modes before `pos`, Wb over the modes after it
+// (column-major factors).  P has leading dimension MS.
+__device__ __forceinline__ double w_after(const Side& sd, int pos, int64_t b, int r) {
+    double w = 1.0;
+    int64_t t = b;
+    for (int q = pos + 1; q < sd.nm; ++q) {
+        const int64_t d = t % sd.dim[q];
+        t /= sd.dim[q];
+        w *= sd.fac[q][d + sd.dim[q] * r];
+    }
+    return w;
+}
+__device__ __forceinline__ double w_before(const Side& sd, int pos, int64_t a, int r) {
+    double w = 1.0;
+    int64_t t = a;
+    for (int q = 0; q < pos; ++q) {
+        const int64_t d = t % sd.dim[q];
+        t /= sd.dim[q];
+        w *= sd.fac[q][d + sd.dim[q] * r];
+    }
+    return w;
+}
+
+// A == 1 (the mode is the side's fastest): thread per (i, r), split over b.
+__global__ __launch_bounds__(256) void k_ttv_lead(const double* __restrict__ P, int64_t MS,
+                                                  int64_t In, int R, Side sd, int pos, int64_t B,
+                                                  int64_t b_per_split, double* __restrict__ out,
+                                                  int64_t split_stride,
+                                                  const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= In * R) return;
+    const int64_t i = idx % In;
+    const int r = (int)(idx / In);
+    const int64_t b0 = (int64_t)blockIdx.y * b_per_split;
+    const int64_t b1 = b0 + b_per_split < B ? b0 + b_per_split : B;
+    const double* __restrict__ Pc = P + MS * r + i;
+    double sum = 0.0;
+    if (sd.nm - pos - 1 == 1) {
+        const double* __restrict__ f = sd.fac[pos + 1] + sd.dim[pos + 1] * r;
+        for (int64_t b = b0; b < b1; ++b) sum += Pc[In * b] * f[b];
+    } else {
+        for (int64_t b = b0; b < b1; ++b) sum += Pc[In * b] * w_after(sd, pos, b, r);
+    }
+    out[(int64_t)blockIdx.y * split_stride + i + In * r] = sum;
+}
+
+// A > 1: one wave per (i, r); lanes stride over a (contiguous in P).
+__global__ __launch_bounds__(256) void k_ttv_inner(const double* __restrict__ P, int64_t MS,
+                                                   int64_t In, int R, Side sd, int pos, int64_t A,
+                                                   int64_t B, double* __restrict__ out,
+                                                   const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= In * R) return;
+    const int64_t i = wid % In;
+    const int r = (int)(wid / In);
+    double sum = 0.0;
+    const bool one_before = (pos == 1);
+    const double* __restrict__ f0 = sd.fac[0] + sd.dim[0] * r;
+    for (int64_t b = 0; b < B; ++b) {
+        const double wb = w_after(sd, pos, b, r);
+        const double* __restrict__ Pc = P + MS * r + A * i + A * In * b;
+        double part = 0.0;
+        if (one_before) {
+            for (int64_t a = lane; a < A; a += 64) part += Pc[a] * f0[a];
+        } else {
+            for (int64_t a = lane; a < A; a += 64) part += Pc[a] * w_before(sd, pos, a, r);
+        }
+        sum += part * wb;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    if (lane == 0) out[i + In * r] = sum;
+}
+
+// out[x] = sum_{z < nz} parts[z*stride + x] (fixed order)
+__global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ parts, int nz,
+                                                   int64_t stride, int64_t n,
+                                                   double* __restrict__ out,
+                                                   const DevCheck* __restrict__ chk) {
+    if (chk && chk->done) return;
+    const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (x >= n) return;
+    double s = 0.0;
+    for (int z = 0; z < nz; ++z) s += parts[(int64_t)z * stride + x];
+    out[x] = s;
+}
+
+// Sum of squares of T: per-block partials, then one block in fixed order.
+template <typename TT>
+__global__ __launch_bounds__(256) void k_sumsq_part(const TT* __restrict__ T, int64_t n,
+                                                    double* __restrict__ part) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * 256) {
+        const double v = (double)T[x];
+        s += v * v;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void k_sum_small(const double* __restrict__ part, int n, double* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += part[i];
+        out[0] = s;
+    }
+}
+
+// ----------------------------------------------------------------- launchers
+namespace {
+
+// contiguous run per lane in the TR kernel (reduction indices per 4-lane group = 4 * TR_RUN)
+constexpr int TR_RUN = 2;
+
+struct Timed {
+    itcpd_plan* p;
+    hipEvent_t e1 = nullptr;
+    Timed(itcpd_plan* pp, double flops, double bytes) : p(pp) {
+        if (!p->profiling) return;
+        hipEvent_t e0;
+        if (hipEventCreate(&e0) != hipSuccess) return;
+        if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); e1 = nullptr; return; }
+        (void)hipEventRecord(e0, p->stream);
+        p->timer.ev.push_back(e0);
+        p->timer.pending_flops.push_back(flops);
+        p->timer.pending_bytes.push_back(bytes);
+    }
+    ~Timed() {
+        if (!e1) return;
+        (void)hipEventRecord(e1, p->stream);
+        p->timer.ev.push_back(e1);
+    }
+};
+
+Side make_side(itcpd_plan* p, int first, int nm, bool rank_fast) {
+    Side sd{};
+    sd.nm = nm;
+    sd.first = first;
+    for (int q = 0; q < nm; ++q) {
+        sd.dim[q] = p->dims[first + q];
+        sd.fac[q] = rank_fast ? p->At[first + q] : p->A[first + q];
+    }
+    return sd;
+}
+
+// Chooses the split-K factor so the grid has roughly `target` workgroups.
+int64_t pick_split(int64_t base_wgs, int64_t kk_total, int64_t target, int64_t min_kk) {
+    int64_t nz = 1;
+    while (base_wgs * nz < target && kk_total / (nz * 2) >= min_kk) nz *= 2;
+    return nz;
+}
+
+template <typename TT, int RB, int MB>
+int run_tk(itcpd_plan* p, const Side& sd, double* out, int64_t out_ld, int64_t nz,
+           int64_t kk_per, int64_t kk_total, int64_t nsteps_i, int64_t split_stride) {
+    constexpr int U = 2;
+    dim3 grid((unsigned)cdiv(p->ML, 64 * MB), (unsigned)(p->Rp / (16 * RB)), (unsigned)nz);
+    const bool fast = (p->ML % (64 * MB) == 0) && (sd.dim[0] % 4 == 0) && (p->ML % 2 == 0);
+    if (fast)
+        hipLaunchKernelGGL((k_tk<TT, RB, MB, U, true>), grid, dim3(256), 0, p->stream,
+                           (const TT*)p->T, p->ML, sd, nsteps_i, kk_per, kk_total, (int)p->Rp,
+                           (int)p->R, out, out_ld, split_stride, p->chk);
+    else
+        hipLaunchKernelGGL((k_tk<TT, RB, MB, U, false>), grid, dim3(256), 0, p->stream,
+                           (const TT*)p->T, p->ML, sd, nsteps_i, kk_per, kk_total, (int)p->Rp,
+                           (int)p->R, out, out_ld, split_stride, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+template <typename TT, int RB, int MB>
+int run_tr(itcpd_plan* p, const Side& sd, double* out, int64_t nz, int64_t kk_per,
+           int64_t kk_total, int64_t ngroups, int64_t split_stride) {
+    constexpr int S = TR_RUN;
+    dim3 grid((unsigned)cdiv(p->MR, 64 * MB), (unsigned)(p->Rp / (16 * RB)), (unsigned)nz);
+    const bool fast = (p->MR % (64 * MB) == 0) && (sd.dim[0] % (4 * S) == 0) && (p->ML % 2 == 0);
+    if (fast)
+        hipLaunchKernelGGL((k_tr<TT, RB, MB, S, true>), grid, dim3(256), 0, p->stream,
+                           (const TT*)p->T, p->ML, p->MR, sd, ngroups, kk_per, kk_total,
+                           (int)p->Rp, (int)p->R, out, split_stride, p->chk);
+    else
+        hipLaunchKernelGGL((k_tr<TT, RB, MB, S, false>), grid, dim3(256), 0, p->stream,
+                           (const TT*)p->T, p->ML, p->MR, sd, ngroups, kk_per, kk_total,
+                           (int)p->Rp, (int)p->R, out, split_stride, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+template <typename TT>
+int dispatch_tk(itcpd_plan* p, const Side& sd, double* out, int64_t out_ld, int64_t nz,
+                int64_t kk_per, int64_t kk_total, int64_t nsteps_i, int64_t split_stride) {
+    switch (p->RB) {
+        case 1: return run_tk<TT, 1, 8>(p, sd, out, out_ld, nz, kk_per, kk_total, nsteps_i, split_stride);
+        case 2: return run_tk<TT, 2, 8>(p, sd, out, out_ld, nz, kk_per, kk_total, nsteps_i, split_stride);
+        case 4: return run_tk<TT, 4, 4>(p, sd, out, out_ld, nz, kk_per, kk_total, nsteps_i, split_stride);
+        default: return run_tk<TT, 8, 2>(p, sd, out, out_ld, nz, kk_per, kk_total, nsteps_i, split_stride);
+    }
+}
+template <typename TT>
+int dispatch_tr(itcpd_plan* p, const Side& sd, double* out, int64_t nz, int64_t kk_per,
+                int64_t kk_total, int64_t ngroups, int64_t split_stride) {
+    switch (p->RB) {
+        case 1: return run_tr<TT, 1, 8>(p, sd, out, nz, kk_per, kk_total, ngroups, split_stride);
+        case 2: return run_tr<TT, 2, 8>(p, sd, out, nz, kk_per, kk_total, ngroups, split_stride);
+        case 4: return run_tr<TT, 4, 4>(p, sd, out, nz, kk_per, kk_total, ngroups, split_stride);
+        default: return run_tr<TT, 8, 2>(p, sd, out, nz, kk_per, kk_total, ngroups, split_stride);
+    }
+}
+
+int mb_of(int RB) { return RB <= 2 ? 8 : RB == 4 ? 4 : 2; }
+
+int sum_parts(itcpd_plan* p, const double* parts, int nz, int64_t stride, int64_t n, double* out) {
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, p->stream, parts,
+                       nz, stride, n, out, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+}  // namespace
+
+// Left GEMM: rows = left multi-index (ML), reduction = right side.  Writes PL
+// (s > 1) or the M of mode 0 into p->M (s == 1).
+int launch_left_gemm(itcpd_plan* p) {
+    const Side sd = make_side(p, p->s, p->N - p->s, /*rank_fast=*/true);
+    const int MB = mb_of(p->RB);
+    const int64_t nsteps_i = cdiv(sd.dim[0], 4);
+    const int64_t nrest = p->MR / sd.dim[0];
+    const int64_t kk_total = nsteps_i * nrest;
+    const int64_t base = cdiv(p->ML, 64 * MB) * (p->Rp / (16 * p->RB));
+    int64_t nz = pick_split(base, kk_total, 1024, 16);
+    const int64_t out_elems = p->ML * p->R;
+    if (nz > 1 && (size_t)(nz * out_elems) > p->parts_elems) nz = (int64_t)(p->parts_elems / out_elems);
+    if (nz < 1) nz = 1;
+    const int64_t kk_per = cdiv(kk_total, nz);
+    nz = cdiv(kk_total, kk_per);
+    double* dst = (p->s > 1) ? p->PL : p->M;
+    double* out = nz > 1 ? p->parts : dst;
+    int rc;
+    {
+        Timed tm(p, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
+                 (double)p->ML * (double)p->MR * p->elt_bytes);
+        rc = (p->elt_bytes == 8)
+                 ? dispatch_tk<double>(p, sd, out, p->ML, nz, kk_per, kk_total, nsteps_i, out_elems)
+                 : dispatch_tk<float>(p, sd, out, p->ML, nz, kk_per, kk_total, nsteps_i, out_elems);
+    }
+    if (rc) return rc;
+    if (nz > 1) return sum_parts(p, p->parts, (int)nz, out_elems, out_elems, dst);
+    return ITCPD_OK;
+}
+
+// Right GEMM: rows = right multi-index (MR), reduction = left side.  Writes PR
+// (N-s > 1) or the M of mode N-1 into p->M (N-s == 1).
+int launch_right_gemm(itcpd_plan* p) {
+    const Side sd = make_side(p, 0, p->s, /*rank_fast=*/false);
+    const int MB = mb_of(p->RB);
+    constexpr int S = TR_RUN;
+    const int64_t ngroups = cdiv(sd.dim[0], 4 * S);
+    const int64_t nrest = p->ML / sd.dim[0];
+    const int64_t kk_total = ngroups * nrest;
+    const int64_t base = cdiv(p->MR, 64 * MB) * (p->Rp / (16 * p->RB));
+    int64_t nz = pick_split(base, kk_total, 512, 8);
+    const int64_t out_elems = p->MR * p->R;
+    if (nz > 1 && (size_t)(nz * out_elems) > p->parts_elems) nz = (int64_t)(p->parts_elems / out_elems);
+    if (nz < 1) nz = 1;
+    const int64_t kk_per = cdiv(kk_total, nz);
+    nz = cdiv(kk_total, kk_per);
+    double* dst = (p->N - p->s > 1) ? p->PR : p->M;
+    double* out = nz > 1 ? p->parts : dst;
+    int rc;
+    {
+        Timed tm(p, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
+                 (double)p->ML * (double)p->MR * p->elt_bytes);
+        rc = (p->elt_bytes == 8)
+                 ? dispatch_tr<double>(p, sd, out, nz, kk_per, kk_total, ngroups, out_elems)
+                 : dispatch_tr<float>(p, sd, out, nz, kk_per, kk_total, ngroups, out_elems);
+    }
+    if (rc) return rc;
+    if (nz > 1) return sum_parts(p, p->parts, (int)nz, out_elems, out_elems, dst);
+    return ITCPD_OK;
+}
+
+// M of `mode` from its side's partial (only for sides with more than one mode).
+int launch_side_ttv(itcpd_plan* p, int mode) {
+    const bool left = mode < p->s;
+    const int first = left ? 0 : p->s;
+    const int nm = left ? p->s : p->N - p->s;
+    const Side sd = make_side(p, first, nm, /*rank_fast=*/false);
+    const double* P = left ? p->PL : p->PR;
+    const int64_t MS = left ? p->ML : p->MR;
+    const int pos = mode - first;
+    int64_t A = 1, B = 1;
+    for (int q = 0; q < pos; ++q) A *= sd.dim[q];
+    for (int q = pos + 1; q < nm; ++q) B *= sd.dim[q];
+    const int64_t In = sd.dim[pos];
+    const int64_t nout = In * p->R;
+    if (A == 1) {
+        int64_t nz = 1;
+        const int64_t blocks = cdiv(nout, 256);
+        while (blocks * nz < 2048 && B / (nz * 2) >= 16 && (size_t)(nz * 2 * nout) <= p->parts_elems)
+            nz *= 2;
+        const int64_t bper = cdiv(B, nz);
+        nz = cdiv(B, bper);
+        double* out = nz > 1 ? p->parts : p->M;
+        hipLaunchKernelGGL(k_ttv_lead, dim3((unsigned)blocks, (unsigned)nz), dim3(256), 0, p->stream,
+                           P, MS, In, (int)p->R, sd, pos, B, bper, out, nout, p->chk);
+        ITCPD_HIP(hipGetLastError());
+        if (nz > 1) return sum_parts(p, p->parts, (int)nz, nout, nout, p->M);
+        return ITCPD_OK;
+    }
+    hipLaunchKernelGGL(k_ttv_inner, dim3((unsigned)cdiv(nout, 4)), dim3(256), 0, p->stream, P, MS,
+                       In, (int)p->R, sd, pos, A, B, p->M, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+// Full MTTKRP of one mode from the current factors, into p->M (hook entry point).
+int mttkrp_mode(itcpd_plan* p, int mode) {
+    int rc;
+    if (mode < p->s) {
+        if ((rc = launch_left_gemm(p))) return rc;
+        if (p->s > 1) return launch_side_ttv(p, mode);
+        return ITCPD_OK;
+    }
+    if ((rc = launch_right_gemm(p))) return rc;
+    if (p->N - p->s > 1) return launch_side_ttv(p, mode);
+    return ITCPD_OK;
+}
+
+int launch_tensor_norm(itcpd_plan* p) {
+    int64_t n = 1;
+    for (int q = 0; q < p->N; ++q) n *= p->dims[q];
+    const int nb = 1024;
+    if (p->elt_bytes == 8)
+        hipLaunchKernelGGL(k_sumsq_part<double>, dim3(nb), dim3(256), 0, p->stream,
+                           (const double*)p->T, n, p->parts);
+    else
+        hipLaunchKernelGGL(k_sumsq_part<float>, dim3(nb), dim3(256), 0, p->stream,
+                           (const float*)p->T, n, p->parts);
+    ITCPD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_small, dim3(1), dim3(64), 0, p->stream, p->parts, nb, p->scal + 3);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+}  // namespace itcpd

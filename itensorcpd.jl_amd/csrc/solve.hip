@@ -1,0 +1,1022 @@
+// Per-mode update on the device: the R x R normal-equation solve, row_norm,
+// the Gram refresh, and (after the last mode) the fit and the convergence check.
+//
+// Reference semantics restated here:
+//  * solve_ls_problem -> ldiv_solve!! (algebra/ldiv_solve.jl:13-29):
+//    cholesky(Hermitian(V), RowMaximum(), check=true, tol=1e-6) \ M^T, i.e. LAPACK
+//    xPSTRF (upper) + xPOTRS; when the pivoted Cholesky stops early (rank < R)
+//    the bare `catch` falls back to qr(V, ColumnNorm()) \ M^T, whose Julia
+//    implementation is the xGELSY scheme: xGEQP3, rank by incremental condition
+//    estimation (xLAIC1) at rcond = R*eps, complete orthogonal decomposition
+//    (xTZRZF), Q^T (xORMQR), triangular solve, Z^T (xORMRZ), un-permute.
+//    (LAPACK 3.x as shipped with OpenBLAS 0.3.x is the third-party code the
+//    reference calls; its published algorithms are restated, not linked.)
+//  * row_norm (math_tools/row_norm.jl:4-24): lambda_r = sqrt(sum_i X(i,r)^2),
+//    A = X ./ lambda.
+//  * post_solve (algorithms/als_algorithms/standard/tensor.jl:46-49): G_n = A_n^T A_n.
+//  * check_converge (converge_checks/fit_check.jl:24-66, no_check.jl:8-20,
+//    converge_checks.jl:5-11).
+#include "internal.h"
+
+namespace itcpd {
+
+struct GramSet {
+    const double* G[ITCPD_MAX_ORDER];
+    int N;
+};
+
+static constexpr int FACTOR_THREADS = 512;
+
+// --------------------------------------------------------- block reductions
+template <int NT>
+__device__ double block_sum(double v, double* red) {
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int o = NT / 2; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// First index of the maximum of f(i) over [lo, hi) (ties -> smaller index).
+// NaN entries never win (they compare false), matching a strict '>' scan.
+template <int NT, typename F>
+__device__ int block_argmax(int lo, int hi, F f, double* redv, int* redi) {
+    const int t = threadIdx.x;
+    double best = -__builtin_inf();
+    int bi = hi;
+    for (int i = lo + t; i < hi; i += NT) {
+        const double v = f(i);
+        if (v > best) {
+            best = v;
+            bi = i;
+        }
+    }
+    redv[t] = best;
+    redi[t] = bi;
+    __syncthreads();
+    for (int o = NT / 2; o > 0; o >>= 1) {
+        if (t < o) {
+            const double v2 = redv[t + o];
+            const int i2 = redi[t + o];
+            if (v2 > redv[t] || (v2 == redv[t] && i2 < redi[t])) {
+                redv[t] = v2;
+                redi[t] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    const int r = redi[0];
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------ xLAIC1 restatement
+// One step of incremental condition estimation (Bischof): given the estimate
+// `sest` of the extreme singular value of an upper-triangular R_j with
+// approximate singular vector x, the new column w (alpha = x^T w) and the new
+// diagonal gamma, returns the estimate for R_{j+1} and the rotation (s, c) that
+// extends x.  job = 1: largest, job = 2: smallest singular value.
+__device__ void laic1(int job, double alpha, double sest, double gamma, double* sestpr, double* s,
+                      double* c) {
+    const double eps = 1.1102230246251565e-16;  // LAPACK dlamch('Epsilon') = 2^-53
+    const double absalp = fabs(alpha), absgam = fabs(gamma), absest = fabs(sest);
+    if (job == 1) {
+        if (sest == 0.0) {
+            const double s1 = fmax(absgam, absalp);
+            if (s1 == 0.0) { *s = 0.0; *c = 1.0; *sestpr = 0.0; return; }
+            double ss = alpha / s1, cc = gamma / s1;
+            const double tmp = sqrt(ss * ss + cc * cc);
+            *s = ss / tmp; *c = cc / tmp; *sestpr = s1 * tmp;
+            return;
+        }
+        if (absgam <= eps * absest) {
+            *s = 1.0; *c = 0.0;
+            const double tmp = fmax(absest, absalp);
+            const double s1 = absest / tmp, s2 = absalp / tmp;
+            *sestpr = tmp * sqrt(s1 * s1 + s2 * s2);
+            return;
+        }
+        if (absalp <= eps * absest) {
+            if (absgam <= absest) { *s = 1.0; *c = 0.0; *sestpr = absest; }
+            else { *s = 0.0; *c = 1.0; *sestpr = absgam; }
+            return;
+        }
+        if (absest <= eps * absalp || absest <= eps * absgam) {
+            if (absgam <= absalp) {
+                const double tmp = absgam / absalp;
+                const double ss = sqrt(1.0 + tmp * tmp);
+                *sestpr = absalp * ss;
+                *c = (gamma / absalp) / ss;
+                *s = copysign(1.0, alpha) / ss;
+            } else {
+                const double tmp = absalp / absgam;
+                const double cc = sqrt(1.0 + tmp * tmp);
+                *sestpr = absgam * cc;
+                *s = (alpha / absgam) / cc;
+                *c = copysign(1.0, gamma) / cc;
+            }
+            return;
+        }
+        const double z1 = alpha / absest, z2 = gamma / absest;
+        const double b = (1.0 - z1 * z1 - z2 * z2) * 0.5;
+        const double cc = z1 * z1;
+        const double t = (b > 0.0) ? cc / (b + sqrt(b * b + cc)) : sqrt(b * b + cc) - b;
+        const double sine = -z1 / t, cosine = -z2 / (1.0 + t);
+        const double tmp = sqrt(sine * sine + cosine * cosine);
+        *s = sine / tmp; *c = cosine / tmp;
+        *sestpr = sqrt(t + 1.0) * absest;
+        return;
+    }
+    // job == 2: smallest singular value
+    if (sest == 0.0) {
+        *sestpr = 0.0;
+        double sine, cosine;
+        if (fmax(absgam, absalp) == 0.0) { sine = 1.0; cosine = 0.0; }
+        else { sine = -gamma; cosine = alpha; }
+        const double s1 = fmax(fabs(sine), fabs(cosine));
+        double ss = sine / s1, cc = cosine / s1;
+        const double tmp = sqrt(ss * ss + cc * cc);
+        *s = ss / tmp; *c = cc / tmp;
+        return;
+    }
+    if (absgam <= eps * absest) { *s = 0.0; *c = 1.0; *sestpr = absgam; return; }
+    if (absalp <= eps * absest) {
+        if (absgam <= absest) { *s = 0.0; *c = 1.0; *sestpr = absgam; }
+        else { *s = 1.0; *c = 0.0; *sestpr = absest; }
+        return;
+    }
+    if (absest <= eps * absalp || absest <= eps * absgam) {
+        if (absgam <= absalp) {
+            const double tmp = absgam / absalp;
+            const double cc = sqrt(1.0 + tmp * tmp);
+            *sestpr = absest * (tmp / cc);
+            *s = -(gamma / absalp) / cc;
+            *c = copysign(1.0, alpha) / cc;
+        } else {
+            const double tmp = absalp / absgam;
+            const double ss = sqrt(1.0 + tmp * tmp);
+            *sestpr = absest / ss;
+            *c = (alpha / absgam) / ss;
+            *s = -copysign(1.0, gamma) / ss;
+        }
+        return;
+    }
+    const double z1 = alpha / absest, z2 = gamma / absest;
+    const double norma = fmax(1.0 + z1 * z1 + fabs(z1 * z2), fabs(z1 * z2) + z2 * z2);
+    const double test = 1.0 + 2.0 * (z1 - z2) * (z1 + z2);
+    double sine, cosine;
+    if (test >= 0.0) {
+        const double b = (z1 * z1 + z2 * z2 + 1.0) * 0.5;
+        const double cc = z2 * z2;
+        const double t = cc / (b + sqrt(fabs(b * b - cc)));
+        sine = z1 / (1.0 - t);
+        cosine = -z2 / t;
+        *sestpr = sqrt(t + 4.0 * eps * eps * norma) * absest;
+    } else {
+        const double b = (z2 * z2 + z1 * z1 - 1.0) * 0.5;
+        const double cc = z1 * z1;
+        const double t = (b >= 0.0) ? -cc / (b + sqrt(b * b + cc)) : b - sqrt(b * b + cc);
+        sine = -z1 / t;
+        cosine = -z2 / (1.0 + t);
+        *sestpr = sqrt(1.0 + t + 4.0 * eps * eps * norma) * absest;
+    }
+    const double tmp = sqrt(sine * sine + cosine * cosine);
+    *s = sine / tmp; *c = cosine / tmp;
+}
+
+// ------------------------------------------------------------ factorisation
+// One workgroup.  Builds V = Hadamard of the Grams of modes != mode, factors it
+// with pivoted Cholesky (upper, right-looking, pivot = largest remaining diagonal,
+// first index on ties; stop when the pivot is <= tol, or <= 0 / NaN at the first
+// step).  Full rank: U (row-major, ld R) and piv are written, finfo[1] = 0.
+// Otherwise the column-pivoted QR solve data are prepared in W (column-major):
+// Householder vectors below the diagonal, T11 and the RZ vectors above it, with
+// qtau, ztau, jpvt, and finfo[2] = numerical rank; finfo[1] = 1.
+__global__ __launch_bounds__(FACTOR_THREADS) void k_factor(GramSet gs, int mode, int R, double tol,
+                                                           double* __restrict__ U,
+                                                           int* __restrict__ piv,
+                                                           int* __restrict__ finfo,
+                                                           double* __restrict__ W,
+                                                           double* __restrict__ qtau,
+                                                           double* __restrict__ ztau,
+                                                           int* __restrict__ jpvt, double* vwork,
+                                                           int use_lds,
+                                                           const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    extern __shared__ double lds[];
+    __shared__ double redv[FACTOR_THREADS];
+    __shared__ int redi[FACTOR_THREADS];
+    const int t = threadIdx.x;
+    constexpr int NT = FACTOR_THREADS;
+    double* S = use_lds ? lds : U;  // full symmetric R x R, row-major
+    int* pv = piv;
+
+    // V = ones .* G_m for m != mode, in mode order (MttkrpAlgorithm.jl:38-47)
+    for (int x = t; x < R * R; x += NT) {
+        double v = 1.0;
+        for (int m = 0; m < gs.N; ++m)
+            if (m != mode) v *= gs.G[m][x];
+        S[x] = v;
+    }
+    for (int i = t; i < R; i += NT) pv[i] = i;
+    __syncthreads();
+
+    int rank = R;
+    for (int j = 0; j < R; ++j) {
+        const int p = block_argmax<NT>(j, R, [&](int i) { return S[i * R + i]; }, redv, redi);
+        const double d = (p < R) ? S[p * R + p] : __builtin_nan("");
+        const bool bad = (j == 0) ? !(d > 0.0) : !(d > tol);
+        if (bad) { rank = j; break; }
+        if (p != j) {
+            // symmetric swap of indices j and p in the trailing block, and of
+            // columns j and p in the finished rows 0..j-1 of U
+            for (int k = t; k < R; k += NT) {
+                if (k >= j) {
+                    const double a = S[j * R + k];
+                    S[j * R + k] = S[p * R + k];
+                    S[p * R + k] = a;
+                }
+            }
+            __syncthreads();
+            for (int k = t; k < R; k += NT) {
+                const double a = S[k * R + j];
+                S[k * R + j] = S[k * R + p];
+                S[k * R + p] = a;
+            }
+            if (t == 0) {
+                const int q = pv[j];
+                pv[j] = pv[p];
+                pv[p] = q;
+            }
+            __syncthreads();
+        }
+        const double ujj = sqrt(S[j * R + j]);
+        const double inv = 1.0 / ujj;
+        __syncthreads();
+        if (t == 0) S[j * R + j] = ujj;
+        for (int k = j + 1 + t; k < R; k += NT) S[j * R + k] *= inv;
+        __syncthreads();
+        // trailing update of both triangles: S(k,l) -= U(j,k) U(j,l)
+        const int n = R - j - 1;
+        for (int x = t; x < n * n; x += NT) {
+            const int k = j + 1 + x / n, l = j + 1 + x % n;
+            S[k * R + l] -= S[j * R + k] * S[j * R + l];
+        }
+        __syncthreads();
+    }
+
+    if (rank == R) {
+        if (use_lds)
+            for (int x = t; x < R * R; x += NT) U[x] = S[x];
+        if (t == 0) { finfo[0] = R; finfo[1] = 0; finfo[2] = R; }
+        return;
+    }
+
+    // ---------------- fallback: column-pivoted QR minimum-norm solve data ----
+    if (t == 0) { finfo[0] = rank; finfo[1] = 1; }
+    // W = V (column-major), recomputed from the Grams
+    for (int x = t; x < R * R; x += NT) {
+        double v = 1.0;
+        for (int m = 0; m < gs.N; ++m)
+            if (m != mode) v *= gs.G[m][x];
+        W[(x % R) + R * (x / R)] = v;   // V symmetric: same element either way
+    }
+    for (int i = t; i < R; i += NT) jpvt[i] = i;
+    __syncthreads();
+    double* vn1 = vwork;
+    double* vn2 = vwork + R;
+    // initial column norms
+    for (int j = t / 64; j < R; j += NT / 64) {
+        double ss = 0.0;
+        for (int k = (t & 63); k < R; k += 64) ss += W[k + R * j] * W[k + R * j];
+        for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+        if ((t & 63) == 0) { vn1[j] = sqrt(ss); vn2[j] = sqrt(ss); }
+    }
+    __syncthreads();
+    const double tol3z = 1.0536712127723509e-08;  // sqrt(dlamch('Epsilon')) = sqrt(2^-53)
+    for (int i = 0; i < R; ++i) {
+        const int p = block_argmax<NT>(i, R, [&](int j) { return fabs(vn1[j]); }, redv, redi);
+        const int pvt = (p < R) ? p : i;
+        if (pvt != i) {
+            for (int k = t; k < R; k += NT) {
+                const double a = W[k + R * pvt];
+                W[k + R * pvt] = W[k + R * i];
+                W[k + R * i] = a;
+            }
+            if (t == 0) {
+                const int q = jpvt[pvt]; jpvt[pvt] = jpvt[i]; jpvt[i] = q;
+                vn1[pvt] = vn1[i];
+                vn2[pvt] = vn2[i];
+            }
+        }
+        __syncthreads();
+        // Householder reflector for W(i:R, i) (xLARFG)
+        double xs = 0.0;
+        for (int k = i + 1 + t; k < R; k += NT) xs += W[k + R * i] * W[k + R * i];
+        const double xnorm2 = block_sum<NT>(xs, redv);
+        const double alpha = W[i + R * i];
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (xnorm2 > 0.0) {
+            const double xnorm = sqrt(xnorm2);
+            beta = -copysign(hypot(alpha, xnorm), alpha);
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        __syncthreads();
+        if (xnorm2 > 0.0)
+            for (int k = i + 1 + t; k < R; k += NT) W[k + R * i] *= scal;
+        __syncthreads();
+        if (t == 0) { W[i + R * i] = beta; qtau[i] = tau; }
+        __syncthreads();
+        // apply H = I - tau v v^T (v = [1; W(i+1:R, i)]) to columns i+1..R-1
+        if (tau != 0.0) {
+            for (int j = i + 1 + t / 64; j < R; j += NT / 64) {
+                double sdot = 0.0;
+                const int ln = t & 63;
+                if (ln == 0) sdot = W[i + R * j];
+                for (int k = i + 1 + ln; k < R; k += 64) sdot += W[k + R * i] * W[k + R * j];
+                for (int o = 32; o > 0; o >>= 1) sdot += __shfl_xor(sdot, o);
+                const double f = tau * sdot;
+                if (ln == 0) W[i + R * j] -= f;
+                for (int k = i + 1 + ln; k < R; k += 64) W[k + R * j] -= f * W[k + R * i];
+            }
+        }
+        __syncthreads();
+        // partial column-norm downdate (xLAQP2)
+        for (int j = i + 1 + t / 64; j < R; j += NT / 64) {
+            const int ln = t & 63;
+            const double v1 = vn1[j];
+            if (v1 != 0.0) {
+                double temp = fabs(W[i + R * j]) / v1;
+                temp = 1.0 - temp * temp;
+                temp = fmax(temp, 0.0);
+                const double q = v1 / vn2[j];
+                const double temp2 = temp * q * q;
+                if (temp2 <= tol3z) {
+                    double ss = 0.0;
+                    for (int k = i + 1 + ln; k < R; k += 64) ss += W[k + R * j] * W[k + R * j];
+                    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+                    if (ln == 0) { vn1[j] = sqrt(ss); vn2[j] = vn1[j]; }
+                } else if (ln == 0) {
+                    vn1[j] = v1 * sqrt(temp);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // numerical rank by incremental condition estimation, rcond = R * eps
+    __shared__ int s_rank;
+    if (t == 0) {
+        const double rcond = (double)R * 2.220446049250313e-16;
+        double* xmin = vwork;       // reuse: vn1/vn2 no longer needed
+        double* xmax = vwork + R;
+        double smax = fabs(W[0]);
+        double smin = smax;
+        int rk;
+        if (smax == 0.0) {
+            rk = 0;
+        } else {
+            rk = 1;
+            xmin[0] = 1.0;
+            xmax[0] = 1.0;
+            while (rk < R) {
+                const int i = rk;
+                double amin = 0.0, amax = 0.0;
+                for (int k = 0; k < i; ++k) {
+                    amin += xmin[k] * W[k + R * i];
+                    amax += xmax[k] * W[k + R * i];
+                }
+                const double gamma = W[i + R * i];
+                double sminpr, s1, c1, smaxpr, s2, c2;
+                laic1(2, amin, smin, gamma, &sminpr, &s1, &c1);
+                laic1(1, amax, smax, gamma, &smaxpr, &s2, &c2);
+                if (smaxpr * rcond <= sminpr) {
+                    for (int k = 0; k < i; ++k) { xmin[k] *= s1; xmax[k] *= s2; }
+                    xmin[i] = c1;
+                    xmax[i] = c2;
+                    smin = sminpr;
+                    smax = smaxpr;
+                    ++rk;
+                } else {
+                    break;
+                }
+            }
+        }
+        s_rank = rk;
+        finfo[2] = rk;
+    }
+    __syncthreads();
+    const int rk = s_rank;
+    if (rk == 0 || rk == R) return;
+
+    // RZ factorisation of the rk x R trapezoid [R11 R12] (xTZRZF / xLATRZ):
+    // for i = rk-1 .. 0, a reflector built from W(i,i) and W(i, rk:R) annihilates
+    // W(i, rk:R) and is applied from the right to rows 0..i-1.
+    const int L = R - rk;
+    for (int i = rk - 1; i >= 0; --i) {
+        double xs = 0.0;
+        for (int q = t; q < L; q += NT) xs += W[i + R * (rk + q)] * W[i + R * (rk + q)];
+        const double xnorm2 = block_sum<NT>(xs, redv);
+        const double alpha = W[i + R * i];
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (xnorm2 > 0.0) {
+            beta = -copysign(hypot(alpha, sqrt(xnorm2)), alpha);
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        __syncthreads();
+        if (xnorm2 > 0.0)
+            for (int q = t; q < L; q += NT) W[i + R * (rk + q)] *= scal;
+        __syncthreads();
+        if (t == 0) { W[i + R * i] = beta; ztau[i] = tau; }
+        __syncthreads();
+        if (tau != 0.0) {
+            // rows k < i: w = C(k,i) + sum_q C(k, rk+q) v_q ; C(k,i) -= tau w ; C(k,rk+q) -= tau w v_q
+            for (int k = t / 64; k < i; k += NT / 64) {
+                const int ln = t & 63;
+                double sdot = (ln == 0) ? W[k + R * i] : 0.0;
+                for (int q = ln; q < L; q += 64) sdot += W[k + R * (rk + q)] * W[i + R * (rk + q)];
+                for (int o = 32; o > 0; o >>= 1) sdot += __shfl_xor(sdot, o);
+                const double f = tau * sdot;
+                if (ln == 0) W[k + R * i] -= f;
+                for (int q = ln; q < L; q += 64) W[k + R * (rk + q)] -= f * W[i + R * (rk + q)];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- row solves
+// X(i, :) = V^{-1} M(i, :) for every row i, one wave per row.  Lane l holds the
+// entries k = l + 64 q (q < RQ).  Also writes per-block partial column sums of
+// squares of X for row_norm.
+template <int RQ>
+__device__ __forceinline__ double bcast(const double (&y)[RQ], int j) {
+    const int q = j >> 6, ln = j & 63;
+    double v = 0.0;
+#pragma unroll
+    for (int qq = 0; qq < RQ; ++qq)
+        if (qq == q) v = y[qq];
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), ln);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), ln);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+template <int RQ>
+__global__ __launch_bounds__(256) void k_solve_rows(const double* __restrict__ M, int64_t In, int R,
+                                                    const double* __restrict__ Ug,
+                                                    const int* __restrict__ piv,
+                                                    const int* __restrict__ finfo,
+                                                    const double* __restrict__ W,
+                                                    const double* __restrict__ qtau,
+                                                    const double* __restrict__ ztau,
+                                                    const int* __restrict__ jpvt, double* __restrict__ X,
+                                                    double* __restrict__ sumsq_part, int u_in_lds,
+                                                    const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    extern __shared__ double lds[];
+    __shared__ double red[4][64 * RQ];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool fallback = finfo[1] != 0;
+    const int ldu = u_in_lds ? R + 1 : R;
+    const double* U = Ug;
+    if (!fallback && u_in_lds) {
+        for (int x = threadIdx.x; x < R * R; x += 256) lds[(x / R) * (R + 1) + (x % R)] = Ug[x];
+        __syncthreads();
+        U = lds;
+    }
+    int pk[RQ];
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+        const int k = lane + 64 * q;
+        pk[q] = (k < R) ? (fallback ? jpvt[k] : piv[k]) : 0;
+    }
+    const int rk = fallback ? finfo[2] : R;
+    double ss[RQ];
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) ss[q] = 0.0;
+
+    for (int64_t i = (int64_t)blockIdx.x * 4 + wave; i < In; i += (int64_t)gridDim.x * 4) {
+        double y[RQ];
+        if (!fallback) {
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int k = lane + 64 * q;
+                y[q] = (k < R) ? M[i + In * pk[q]] : 0.0;
+            }
+            // U^T z = y (forward), then U w = z (backward)
+            for (int j = 0; j < R; ++j) {
+                const double zj = bcast<RQ>(y, j) / U[j * ldu + j];
+#pragma unroll
+                for (int q = 0; q < RQ; ++q) {
+                    const int k = lane + 64 * q;
+                    if (k == j) y[q] = zj;
+                    else if (k > j && k < R) y[q] -= U[j * ldu + k] * zj;
+                }
+            }
+            for (int j = R - 1; j >= 0; --j) {
+                const double wj = bcast<RQ>(y, j) / U[j * ldu + j];
+#pragma unroll
+                for (int q = 0; q < RQ; ++q) {
+                    const int k = lane + 64 * q;
+                    if (k == j) y[q] = wj;
+                    else if (k < j) y[q] -= U[k * ldu + j] * wj;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int k = lane + 64 * q;
+                y[q] = (k < R) ? M[i + In * k] : 0.0;
+            }
+            if (rk == 0) {
+#pragma unroll
+                for (int q = 0; q < RQ; ++q) y[q] = 0.0;
+            } else {
+                // Q^T y: H_0 first (xORMQR, side L, trans T)
+                for (int tq = 0; tq < R; ++tq) {
+                    const double tau = qtau[tq];
+                    if (tau == 0.0) continue;
+                    double part = 0.0;
+#pragma unroll
+                    for (int q = 0; q < RQ; ++q) {
+                        const int k = lane + 64 * q;
+                        if (k == tq) part += y[q];
+                        else if (k > tq && k < R) part += W[k + (int64_t)R * tq] * y[q];
+                    }
+                    const double f = tau * wave_sum(part);
+#pragma unroll
+                    for (int q = 0; q < RQ; ++q) {
+                        const int k = lane + 64 * q;
+                        if (k == tq) y[q] -= f;
+                        else if (k > tq && k < R) y[q] -= f * W[k + (int64_t)R * tq];
+                    }
+                }
+                // T11 y1 = (Q^T y)(0:rk), backward; y(rk:R) = 0
+#pragma unroll
+                for (int q = 0; q < RQ; ++q)
+                    if (lane + 64 * q >= rk) y[q] = 0.0;
+                for (int j = rk - 1; j >= 0; --j) {
+                    const double wj = bcast<RQ>(y, j) / W[j + (int64_t)R * j];
+#pragma unroll
+                    for (int q = 0; q < RQ; ++q) {
+                        const int k = lane + 64 * q;
+                        if (k == j) y[q] = wj;
+                        else if (k < j) y[q] -= W[k + (int64_t)R * j] * wj;
+                    }
+                }
+                // Z^T y: Z_0 first (xORMRZ, side L, trans T)
+                if (rk < R) {
+                    for (int tq = 0; tq < rk; ++tq) {
+                        const double tau = ztau[tq];
+                        if (tau == 0.0) continue;
+                        double part = 0.0;
+#pragma unroll
+                        for (int q = 0; q < RQ; ++q) {
+                            const int k = lane + 64 * q;
+                            if (k == tq) part += y[q];
+                            else if (k >= rk && k < R) part += W[tq + (int64_t)R * k] * y[q];
+                        }
+                        const double f = tau * wave_sum(part);
+#pragma unroll
+                        for (int q = 0; q < RQ; ++q) {
+                            const int k = lane + 64 * q;
+                            if (k == tq) y[q] -= f;
+                            else if (k >= rk && k < R) y[q] -= f * W[tq + (int64_t)R * k];
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int k = lane + 64 * q;
+            if (k < R) {
+                X[i + In * pk[q]] = y[q];
+                ss[q] += y[q] * y[q];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) red[wave][lane + 64 * q] = ss[q];
+    __syncthreads();
+    // block partial indexed by the un-permuted column
+    for (int x = threadIdx.x; x < 64 * RQ; x += 256) {
+        if (x < R) {
+            const double v = red[0][x] + red[1][x] + red[2][x] + red[3][x];
+            const int col = fallback ? jpvt[x] : piv[x];
+            sumsq_part[(int64_t)blockIdx.x * R + col] = v;
+        }
+    }
+}
+
+// ------------------------------------------------ row_norm + Gram partials
+// lambda from the per-block sums of squares; A = X ./ lambda (column-major,
+// Rp columns) and At (rank fastest); Gram partial over this block's rows; for
+// the fit, <M, A .* lambda> over this block's rows.
+__global__ __launch_bounds__(256) void k_normalize(const double* __restrict__ X, int64_t In, int R,
+                                                   int Rp, const double* __restrict__ sumsq_part,
+                                                   int nsp, int rows_per_block, double* __restrict__ A,
+                                                   double* __restrict__ At, double* __restrict__ lam,
+                                                   double* __restrict__ gram_part,
+                                                   const double* __restrict__ M, int want_inner,
+                                                   double* __restrict__ inner_part, int normalize,
+                                                   const DevCheck* __restrict__ chk) {
+    if (chk && chk->done) return;
+    extern __shared__ double lds[];
+    double* lamS = lds;            // R
+    double* As = lds + R;          // rows_per_block x R (row-major)
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    for (int r = t; r < R; r += 256) {
+        if (normalize) {
+            double s = 0.0;
+            for (int b = 0; b < nsp; ++b) s += sumsq_part[(int64_t)b * R + r];
+            lamS[r] = sqrt(s);
+        } else {
+            lamS[r] = 1.0;
+        }
+    }
+    __syncthreads();
+    if (normalize && blockIdx.x == 0)
+        for (int r = t; r < R; r += 256) lam[r] = lamS[r];
+    double inner = 0.0;
+    double* gp = gram_part + (int64_t)blockIdx.x * R * R;
+    bool first_tile = true;
+    for (int64_t i0 = (int64_t)blockIdx.x * rows_per_block; i0 < In;
+         i0 += (int64_t)gridDim.x * rows_per_block) {
+        const int nr = (int)((In - i0) < rows_per_block ? (In - i0) : rows_per_block);
+        for (int x = t; x < nr * R; x += 256) {
+            const int ii = x % nr, r = x / nr;
+            const int64_t i = i0 + ii;
+            const double a = normalize ? X[i + In * r] / lamS[r] : X[i + In * r];
+            As[ii * R + r] = a;
+            if (normalize) A[i + In * r] = a;
+            At[i * Rp + r] = a;
+            if (want_inner) inner += M[i + In * r] * (a * lamS[r]);
+        }
+        __syncthreads();
+        for (int x = t; x < R * R; x += 256) {
+            const int r = x % R, r2 = x / R;
+            double s = 0.0;
+            for (int ii = 0; ii < nr; ++ii) s += As[ii * R + r] * As[ii * R + r2];
+            gp[x] = first_tile ? s : gp[x] + s;
+        }
+        first_tile = false;
+        __syncthreads();
+    }
+    if (want_inner) {
+        const double v = block_sum<256>(inner, red);
+        if (t == 0) inner_part[blockIdx.x] = v;
+    }
+}
+
+// Sums the Gram partials into G_mode; after the last mode also evaluates the fit
+// and advances the check state (FitCheck / NoCheck), recording the fit trace.
+__global__ __launch_bounds__(256) void k_gram_final(const double* __restrict__ gram_part, int nb,
+                                                    int R, double* __restrict__ Gout, GramSet gs,
+                                                    int mode, const double* __restrict__ lam,
+                                                    const double* __restrict__ inner_part,
+                                                    int do_check, double* __restrict__ scal,
+                                                    DevCheck* __restrict__ chk,
+                                                    double* __restrict__ trace, int trace_cap) {
+    if (chk && chk->done) return;
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    for (int x = t; x < R * R; x += 256) {
+        double s = 0.0;
+        for (int b = 0; b < nb; ++b) s += gram_part[(int64_t)b * R * R + x];
+        Gout[x] = s;
+    }
+    if (!do_check) return;
+    __threadfence_block();
+    __syncthreads();
+    if (chk->kind == ITCPD_CHECK_NONE) {
+        if (t == 0) {
+            chk->iter += 1;
+            chk->sweeps += 1;
+            if (chk->iter == chk->max_counter) {
+                chk->iter = 0;
+                chk->done = 1;
+            }
+        }
+        return;
+    }
+    // ||X||^2 = lambda^T (G_0 .* ... .* G_{N-1}) lambda   (converge_checks.jl:5-11)
+    double fs = 0.0;
+    for (int x = t; x < R * R; x += 256) {
+        const int r = x % R, r2 = x / R;
+        double h = gs.G[0][x];
+        for (int m = 1; m < gs.N; ++m) h *= gs.G[m][x];
+        fs += h * (lam[r] * lam[r2]);
+    }
+    const double fact_square = block_sum<256>(fs, red);
+    if (t == 0) {
+        double inner = 0.0;
+        for (int b = 0; b < nb; ++b) inner += inner_part[b];
+        const double ref = chk->ref_norm;
+        const double normResidual = sqrt(fabs(ref * ref + fact_square - 2.0 * fabs(inner)));
+        const double fit = 1.0 - normResidual / ref;
+        scal[0] = inner;
+        scal[1] = fact_square;
+        scal[2] = fit;
+        if (chk->kind == ITCPD_CHECK_FIT) {
+            chk->iter += 1;
+            const double dfit = fabs(chk->lastfit - fit);
+            chk->lastfit = fit;
+            if (trace && chk->sweeps < trace_cap) trace[chk->sweeps] = fit;
+            chk->sweeps += 1;
+            if (fit != fit) {
+                chk->nan_flag = 1;
+                chk->done = 1;
+                return;
+            }
+            if (dfit < chk->tolerance) {
+                chk->counter += 1;
+                if (chk->counter >= 2) {
+                    chk->total_iter = chk->iter;
+                    chk->iter = 0;
+                    chk->counter = 0;
+                    chk->final_fit = chk->lastfit;
+                    chk->lastfit = 0.0;
+                    chk->done = 1;
+                    return;
+                }
+            } else {
+                chk->counter = 0;
+            }
+            if (chk->iter >= chk->max_counter) {
+                chk->total_iter = chk->iter;
+                chk->iter = 0;
+                chk->counter = 0;
+                chk->final_fit = chk->lastfit;
+                chk->lastfit = 0.0;
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------- init (RNG)
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): counter = (element lo, element
+// hi, mode, 0), key = (seed lo, seed hi).  One N(0,1) value per counter by
+// Box-Muller on two 53-bit uniforms: u1 in (0, 1], u2 in [0, 1).
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+__global__ __launch_bounds__(256) void k_random_normal(double* __restrict__ X, int64_t In, int R,
+                                                       uint64_t seed, int mode) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;   // e = i + In * r
+    if (e >= In * R) return;
+    uint32_t c[4] = {(uint32_t)e, (uint32_t)((uint64_t)e >> 32), (uint32_t)mode, 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t a = ((uint64_t)c[0] << 32) | c[1];
+    const uint64_t b = ((uint64_t)c[2] << 32) | c[3];
+    const double u1 = (double)((a >> 11) + 1) * 1.1102230246251565e-16;  // (0, 1]
+    const double u2 = (double)(b >> 11) * 1.1102230246251565e-16;        // [0, 1)
+    X[e] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+// per-block sums of squares of the columns of X (for the init's row_norm)
+__global__ __launch_bounds__(256) void k_col_sumsq(const double* __restrict__ X, int64_t In, int R,
+                                                   int rows_per_block, double* __restrict__ part) {
+    const int64_t i0 = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t i1 = i0 + rows_per_block < In ? i0 + rows_per_block : In;
+    for (int r = threadIdx.x; r < R; r += 256) {
+        double s = 0.0;
+        for (int64_t i = i0; i < i1; ++i) s += X[i + In * r] * X[i + In * r];
+        part[(int64_t)blockIdx.x * R + r] = s;
+    }
+}
+
+// V = Hadamard of the Grams of modes != mode (compute_krp hook)
+__global__ __launch_bounds__(256) void k_hadamard_grams(GramSet gs, int mode, int R,
+                                                        double* __restrict__ V) {
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= R * R) return;
+    double v = 1.0;
+    for (int m = 0; m < gs.N; ++m)
+        if (m != mode) v *= gs.G[m][x];
+    V[x] = v;
+}
+
+// ----------------------------------------------------------------- fit hook
+__global__ __launch_bounds__(256) void k_fit_hook(const double* __restrict__ M,
+                                                  const double* __restrict__ AN, int64_t In, int R,
+                                                  const double* __restrict__ lam, GramSet gs,
+                                                  double ref, double* __restrict__ scal) {
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    double inner = 0.0;
+    for (int64_t x = t; x < In * R; x += 256) {
+        const int r = (int)(x / In);
+        inner += M[x] * (AN[x] * lam[r]);
+    }
+    inner = block_sum<256>(inner, red);
+    double fs = 0.0;
+    for (int x = t; x < R * R; x += 256) {
+        const int r = x % R, r2 = x / R;
+        double h = gs.G[0][x];
+        for (int m = 1; m < gs.N; ++m) h *= gs.G[m][x];
+        fs += h * (lam[r] * lam[r2]);
+    }
+    fs = block_sum<256>(fs, red);
+    if (t == 0) {
+        scal[0] = inner;
+        scal[1] = fs;
+        scal[2] = 1.0 - sqrt(fabs(ref * ref + fs - 2.0 * fabs(inner))) / ref;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+namespace {
+
+GramSet grams_of(itcpd_plan* p) {
+    GramSet gs{};
+    gs.N = p->N;
+    for (int m = 0; m < p->N; ++m) gs.G[m] = p->G[m];
+    return gs;
+}
+
+int rows_per_block_for(int R) {
+    int rpb = 32;
+    while (rpb > 1 && (size_t)(rpb * R + R) * 8 > 120 * 1024) rpb >>= 1;
+    return rpb;
+}
+
+// Number of row-normalisation blocks: capped so the Gram partials stay bounded.
+int norm_blocks(int64_t In, int R, int rpb) {
+    int64_t nb = cdiv(In, rpb);
+    int64_t cap = (int64_t)1 << 25;
+    cap = cap / ((int64_t)R * R);
+    if (cap > 64) cap = 64;
+    if (cap < 1) cap = 1;
+    return (int)(nb < cap ? nb : cap);
+}
+
+int solve_blocks(int64_t In) {
+    int64_t nb = cdiv(In, 4);
+    return (int)(nb < 128 ? nb : 128);
+}
+
+template <int RQ>
+int run_solve_rows(itcpd_plan* p, int64_t In, const double* Msrc, int nb, bool u_lds) {
+    const size_t lds = u_lds ? (size_t)p->R * (p->R + 1) * 8 : 0;
+    hipLaunchKernelGGL(k_solve_rows<RQ>, dim3(nb), dim3(256), lds, p->stream, Msrc, In, (int)p->R,
+                       p->U, p->piv, p->finfo, p->qr_aux, p->qr_aux + p->R * p->R,
+                       p->qr_aux + p->R * p->R + p->R, (const int*)(p->piv + p->R), p->X,
+                       p->sumsq_part, u_lds ? 1 : 0, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+}  // namespace
+
+int launch_factor(itcpd_plan* p, int mode) {
+    const int R = (int)p->R;
+    const bool lds = (size_t)R * R * 8 <= 128 * 1024;
+    double* W = p->qr_aux;
+    double* qtau = W + (size_t)R * R;
+    double* ztau = qtau + R;
+    double* vwork = ztau + R;
+    hipLaunchKernelGGL(k_factor, dim3(1), dim3(FACTOR_THREADS), lds ? (size_t)R * R * 8 : 0,
+                       p->stream, grams_of(p), mode, R, 1e-6, p->U, p->piv, p->finfo, W, qtau, ztau,
+                       p->piv + R, vwork, lds ? 1 : 0, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+int launch_solve_rows(itcpd_plan* p, int mode, const double* Msrc) {
+    const int64_t In = p->dims[mode];
+    const int nb = solve_blocks(In);
+    const bool u_lds = (size_t)p->R * (p->R + 1) * 8 <= 130 * 1024;
+    const int R = (int)p->R;
+    if (R <= 64) return run_solve_rows<1>(p, In, Msrc, nb, u_lds);
+    if (R <= 128) return run_solve_rows<2>(p, In, Msrc, nb, u_lds);
+    if (R <= 256) return run_solve_rows<4>(p, In, Msrc, nb, false);
+    if (R <= 512) return run_solve_rows<8>(p, In, Msrc, nb, false);
+    if (R <= 1024) return run_solve_rows<16>(p, In, Msrc, nb, false);
+    return run_solve_rows<32>(p, In, Msrc, nb, false);
+}
+
+int launch_normalize(itcpd_plan* p, int mode, const double* Xsrc, bool fit_after) {
+    const int64_t In = p->dims[mode];
+    const int R = (int)p->R;
+    const int rpb = rows_per_block_for(R);
+    const int nb2 = norm_blocks(In, R, rpb);
+    const int nsp = solve_blocks(In);
+    const size_t lds = (size_t)(R + rpb * R) * 8;
+    hipLaunchKernelGGL(k_normalize, dim3(nb2), dim3(256), lds, p->stream, Xsrc, In, R, (int)p->Rp,
+                       p->sumsq_part, nsp, rpb, p->A[mode], p->At[mode], p->lam, p->gram_part,
+                       p->M, fit_after ? 1 : 0, p->inner_part, 1, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(256), 0, p->stream, p->gram_part, nb2, R,
+                       p->G[mode], grams_of(p), mode, p->lam, p->inner_part, fit_after ? 1 : 0,
+                       p->scal, p->chk, p->fit_trace, p->trace_cap);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+int launch_update_mode(itcpd_plan* p, int mode, bool fit_after) {
+    int rc;
+    if ((rc = launch_factor(p, mode))) return rc;
+    if ((rc = launch_solve_rows(p, mode, p->M))) return rc;
+    return launch_normalize(p, mode, p->X, fit_after);
+}
+
+// Gram cache seeding (optimizers/als_optimizers/standard/tensor.jl:12) and the
+// rank-fastest copies of the factors.
+int launch_seed_grams(itcpd_plan* p) {
+    const int R = (int)p->R;
+    const int rpb = rows_per_block_for(R);
+    for (int m = 0; m < p->N; ++m) {
+        const int64_t In = p->dims[m];
+        const int nb2 = norm_blocks(In, R, rpb);
+        const size_t lds = (size_t)(R + rpb * R) * 8;
+        hipLaunchKernelGGL(k_normalize, dim3(nb2), dim3(256), lds, p->stream, p->A[m], In, R,
+                           (int)p->Rp, p->sumsq_part, 0, rpb, p->A[m], p->At[m], p->lam,
+                           p->gram_part, p->M, 0, p->inner_part, 0, (const DevCheck*)nullptr);
+        ITCPD_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(256), 0, p->stream, p->gram_part, nb2, R,
+                           p->G[m], grams_of(p), m, p->lam, p->inner_part, 0, p->scal,
+                           (DevCheck*)nullptr, (double*)nullptr, 0);
+        ITCPD_HIP(hipGetLastError());
+    }
+    return ITCPD_OK;
+}
+
+// random_factors (cpd.jl:48-60): draw, row_norm, lambda from the last mode.
+int launch_random_init(itcpd_plan* p, uint64_t seed) {
+    const int R = (int)p->R;
+    const int rpb = rows_per_block_for(R);
+    for (int m = 0; m < p->N; ++m) {
+        const int64_t In = p->dims[m];
+        hipLaunchKernelGGL(k_random_normal, dim3((unsigned)cdiv(In * R, 256)), dim3(256), 0,
+                           p->stream, p->X, In, R, seed, m);
+        ITCPD_HIP(hipGetLastError());
+        const int nsp = solve_blocks(In);
+        const int per = (int)cdiv(In, nsp);
+        hipLaunchKernelGGL(k_col_sumsq, dim3(nsp), dim3(256), 0, p->stream, p->X, In, R, per,
+                           p->sumsq_part);
+        ITCPD_HIP(hipGetLastError());
+        const int nb2 = norm_blocks(In, R, rpb);
+        const size_t lds = (size_t)(R + rpb * R) * 8;
+        hipLaunchKernelGGL(k_normalize, dim3(nb2), dim3(256), lds, p->stream, p->X, In, R,
+                           (int)p->Rp, p->sumsq_part, nsp, rpb, p->A[m], p->At[m], p->lam,
+                           p->gram_part, p->M, 0, p->inner_part, 1, (const DevCheck*)nullptr);
+        ITCPD_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(256), 0, p->stream, p->gram_part, nb2, R,
+                           p->G[m], grams_of(p), m, p->lam, p->inner_part, 0, p->scal,
+                           (DevCheck*)nullptr, (double*)nullptr, 0);
+        ITCPD_HIP(hipGetLastError());
+    }
+    return ITCPD_OK;
+}
+
+int launch_fit_only(itcpd_plan* p, const double* Mlast, double ref_norm) {
+    const int N = p->N;
+    hipLaunchKernelGGL(k_fit_hook, dim3(1), dim3(256), 0, p->stream, Mlast, p->A[N - 1],
+                       p->dims[N - 1], (int)p->R, p->lam, grams_of(p), ref_norm, p->scal);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+int launch_hadamard(itcpd_plan* p, int mode, double* V) {
+    const int R = (int)p->R;
+    hipLaunchKernelGGL(k_hadamard_grams, dim3((unsigned)cdiv((int64_t)R * R, 256)), dim3(256), 0,
+                       p->stream, grams_of(p), mode, R, V);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+// Column sums of squares of X in the block partition the row solve uses.
+int launch_col_sumsq(itcpd_plan* p, int mode, const double* X) {
+    const int64_t In = p->dims[mode];
+    const int nsp = solve_blocks(In);
+    const int per = (int)cdiv(In, nsp);
+    hipLaunchKernelGGL(k_col_sumsq, dim3(nsp), dim3(256), 0, p->stream, X, In, (int)p->R, per,
+                       p->sumsq_part);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+}  // namespace itcpd

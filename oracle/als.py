@@ -1,0 +1,513 @@
+"""numpy/scipy restatement of ITensorCPD.jl's dense CP-ALS path (test infrastructure).
+
+Only tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py``
+use this module, as the checker.  It is never the thing measured or shipped.
+
+Conventions.  A tensor is an ``ndarray`` of shape ``(I_1, ..., I_N)`` indexed
+``T[i_1, ..., i_N]`` (memory order does not matter here).  A factor is an
+``(I_n, R)`` array, lambda a length-``R`` vector, a Gram an ``(R, R)`` array.
+Modes are 0-based in this file (mode ``n`` here is mode ``n+1`` in Julia).
+
+Arithmetic follows the reference's calls exactly where an arithmetic detail is
+visible at the call site:
+  * pivoted Cholesky = LAPACK ``dpstrf`` on the upper triangle with the absolute
+    tolerance ``cholesky_epsilon = 1e-6`` (``ldiv_solve.jl:18``, ``ITensorCPD.jl:2``);
+  * its fallback ``qr(A, ColumnNorm()) \\ B`` = LAPACK ``dgeqp3`` + incremental
+    condition estimation + complete orthogonal decomposition with
+    ``rcond = min(m, n) * eps``, which is exactly LAPACK ``dgelsy``
+    (``ldiv_solve.jl:21,24``);
+  * ``row_norm`` divides by lambda (``row_norm.jl:21``) and lambda is
+    ``sqrt(sum(t .* t))`` (``row_norm.jl:7,13``).
+
+What pins each function (checks in ``tests/test_oracle.py``):
+  row_norm          unit columns (``basic_features.jl:1-16``) + closed form
+  mttkrp            brute-force loops, KRP-GEMM identity, rank-1 closed form
+  compute_krp/grams brute-force loops
+  ldiv_solve        residual vs numpy solve; pinv on exact-rank systems; the
+                    1e-6 absolute pivot threshold deciding the fallback
+  fit (FitCheck)    explicit ||T - reconstruct|| on small tensors
+  FitCheck/NoCheck  scripted fit sequences through the state machine
+  reconstruct       brute-force loops (``basic_features.jl:60-92``)
+  decompose/optimize the reference's end-to-end bars (``cp_als.jl``)
+  random_factors    distribution/normalisation invariants only; the values of
+                    Julia's ``MersenneTwister(3)`` stream are parity unpinned
+                    (no Julia runtime is available to produce them)
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+from scipy.linalg import lapack
+
+__all__ = [
+    "CHOLESKY_EPSILON", "CPD", "OracleNaN", "row_norm", "random_factors", "random_CPD",
+    "grams", "compute_krp", "mttkrp", "ldiv_solve", "qr_colnorm_solve", "solve_ls_problem",
+    "norm_factors", "FitCheck", "NoCheck", "CPDiffCheck", "CPAngleCheck", "check_converge",
+    "optimize", "als_optimize", "decompose", "decompose_adaptive", "increase_cpd_rank",
+    "reconstruct", "cp_cp_contract", "KRPNormal", "KRPFreeNormal", "fit_from_parts",
+    "fit_state_update", "CPDFit",
+]
+
+# ITensorCPD.jl:2 — absolute pivot tolerance handed to the pivoted Cholesky.
+CHOLESKY_EPSILON = 1e-6
+_EPS64 = float(np.finfo(np.float64).eps)
+
+
+class OracleNaN(RuntimeError):
+    """Mirrors ``throw("Error NAN")`` (fit_check.jl:40-42)."""
+
+
+class KRPFreeNormal:
+    """Per-rank-slice MTTKRP (algorithms/.../standard/tensor.jl:30-49)."""
+
+
+class KRPNormal:
+    """Explicit-KRP MTTKRP (algorithms/.../standard/tensor.jl:10-25)."""
+
+
+@dataclass
+class CPD:
+    """CPD container (cpd.jl:7-11): factors (I_n x R) and lambda (R)."""
+    factors: List[np.ndarray]
+    lam: np.ndarray
+
+    def __getitem__(self, i):
+        return self.factors[i]
+
+    @property
+    def rank(self) -> int:
+        return int(self.lam.shape[0])
+
+    def copy(self) -> "CPD":
+        return CPD([f.copy() for f in self.factors], self.lam.copy())
+
+
+# --------------------------------------------------------------------------- init
+def row_norm(t: np.ndarray):
+    """row_norm.jl:4-24 for a factor ``t(i, r)`` normalised over ``i``.
+
+    lambda_r = sqrt(sum_i t[i,r] * t[i,r]) (``t .* t``, row_norm.jl:7; the
+    ``ones`` contraction sums over i, :9-10; sqrt, :12); returns t ./ lambda
+    (row_norm.jl:19-22) and lambda.
+    """
+    lam = np.sqrt(np.sum(t * t, axis=0))
+    return t / lam[None, :], lam
+
+
+def random_factors(dims, rank: int, rng=None):
+    """cpd.jl:48-60: per mode draw an I_n x R N(0,1) matrix, then ``row_norm``.
+
+    lambda is the column-norm vector of the LAST mode (cpd.jl:55-57).  The
+    reference's default stream is Julia ``MersenneTwister(3)`` (cpd.jl:50); this
+    restatement uses numpy ``default_rng(3)`` — same distribution, different
+    values (parity unpinned: needs a Julia runtime).
+    """
+    rng = np.random.default_rng(3) if rng is None else rng
+    factors, lam = [], None
+    for I in dims:
+        f, lam = row_norm(rng.standard_normal((int(I), int(rank))))
+        factors.append(f)
+    return factors, lam
+
+
+def random_CPD(target, rank: int, rng=None) -> CPD:
+    """cpd.jl:63-82 (``random_CPD(::AbstractArray, rank)``)."""
+    factors, lam = random_factors(np.shape(target), rank, rng)
+    return CPD(factors, lam)
+
+
+# ------------------------------------------------------------------- per-mode hooks
+def grams(factors):
+    """Gram seeding, optimizers/als_optimizers/standard/tensor.jl:12 (A_n^T A_n)."""
+    return [f.T @ f for f in factors]
+
+
+def compute_krp(gram_list, n: int) -> np.ndarray:
+    """MttkrpAlgorithm.jl:18-31: V = ones, then V .*= G_m for every m != n in mode order."""
+    V = np.ones_like(gram_list[0])
+    for m, G in enumerate(gram_list):
+        if m == n:
+            continue
+        V = V * G
+    return V
+
+
+def _contract_vec(x: np.ndarray, v: np.ndarray, axis: int) -> np.ndarray:
+    return np.tensordot(x, v, axes=([axis], [0]))
+
+
+def mttkrp(T: np.ndarray, factors, n: int, form: str = "tree") -> np.ndarray:
+    """M_n(i_n, r) = sum T(i_1..i_N) prod_{m != n} A_m(i_m, r).
+
+    form="krpfree": the reference default, one full contraction of T with N-1
+        vectors per rank slice (had_contract.jl:72-124, hot loop :111-121; called
+        from algorithms/.../standard/tensor.jl:32-44).
+    form="krp": explicit Khatri-Rao product, then one contraction ``target * krp``
+        (algorithms/.../standard/tensor.jl:12-20).
+    form="tree": the same sum computed by BLAS-backed pairwise contractions
+        (identical mathematics, different rounding); used for large sizes.
+    """
+    N = T.ndim
+    R = factors[0].shape[1]
+    others = [m for m in range(N) if m != n]
+    if form == "krpfree":
+        out = np.empty((T.shape[n], R))
+        for r in range(R):
+            x = T
+            # contract the trailing modes first so axis numbers stay valid
+            for m in reversed(others):
+                x = _contract_vec(x, factors[m][:, r], m)
+            out[:, r] = x
+        return out
+    if form == "krp":
+        # KRP(i_others..., r) = prod_m A_m(i_m, r)   (had_contract over the factors)
+        K = np.ones((1, R))
+        shape = []
+        for m in others:
+            K = (K[:, None, :] * factors[m][None, :, :]).reshape(-1, R)
+            shape.append(T.shape[m])
+        # C-order reshape: rows enumerate (i_others[0], ..., i_others[-1]), last fastest
+        K = K.reshape(*shape, R)
+        return np.tensordot(T, K, axes=(others, list(range(len(others)))))
+    if form == "tree":
+        if N == 1:
+            return T[:, None] * np.ones((1, R))
+        # contract the largest other mode by GEMM, then Hadamard-reduce the rest
+        big = max(others, key=lambda m: T.shape[m])
+        P = np.tensordot(T, factors[big], axes=([big], [0]))   # modes (others-big..., n) + r
+        rest = [m for m in range(N) if m != big]              # axis order of P (before r)
+        for m in reversed(rest):
+            if m == n:
+                continue
+            ax = rest.index(m)
+            P = np.einsum(_sub(P.ndim, ax), P, factors[m])
+            rest.pop(ax)
+        return P
+    raise ValueError(form)
+
+
+def _sub(nd: int, ax: int) -> str:
+    letters = "abcdefghijklmnopq"
+    src = letters[:nd - 1] + "z"
+    dst = "".join(c for k, c in enumerate(letters[:nd - 1]) if k != ax) + "z"
+    return f"{src},{letters[ax]}z->{dst}"
+
+
+def qr_colnorm_solve(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """``qr(A, ColumnNorm()) \\ B`` (ldiv_solve.jl:21,24).
+
+    Julia's QRPivoted solve determines the rank with incremental condition
+    estimation at ``rcond = min(m, n) * eps`` and returns the minimum-norm
+    solution through the complete orthogonal decomposition — LAPACK ``dgelsy``.
+    """
+    m, n = A.shape
+    rcond = min(m, n) * _EPS64
+    nrhs = B.shape[1]
+    jpvt = np.zeros(n, dtype=np.int32)
+    lwork = max(min(m, n) + 3 * n + 1, 2 * min(m, n) + nrhs) * 4 + 64
+    b = np.zeros((max(m, n), nrhs))
+    b[:m] = B
+    _, x, _, rank, info = lapack.dgelsy(np.array(A, dtype=np.float64, order="F"), b, jpvt,
+                                        rcond, lwork)
+    if info != 0:
+        raise RuntimeError(f"dgelsy info={info}")
+    return x[:n]
+
+
+def ldiv_solve(A: np.ndarray, B: np.ndarray, tol: float = CHOLESKY_EPSILON):
+    """ldiv_solve.jl:13-29 with ``factorizeA = true``.
+
+    Square A: ``cholesky(Hermitian(A), RowMaximum(), check=true, tol)`` = LAPACK
+    ``dpstrf`` on the upper triangle; ``check=true`` throws when the computed
+    rank is below n, and the bare ``catch`` falls back to the pivoted QR solve.
+    Returns (X, used_fallback).
+    """
+    m, n = A.shape
+    if m == n:
+        c, piv, rank, info = lapack.dpstrf(np.array(A, dtype=np.float64, order="F"),
+                                           tol=tol, lower=0)
+        if rank >= n and info == 0:
+            p = piv - 1                       # LAPACK pivots are 1-based
+            z, info2 = lapack.dpotrs(c, np.ascontiguousarray(B[p, :]), lower=0)
+            if info2 != 0:
+                raise RuntimeError(f"dpotrs info={info2}")
+            x = np.empty_like(z)
+            x[p, :] = z
+            return x, False
+    return qr_colnorm_solve(A, B), True
+
+
+def solve_ls_problem(V: np.ndarray, M: np.ndarray):
+    """MttkrpAlgorithm.jl:34-41: X = ldiv_solve(V, M^T)^T (I_n x R); (X, used_fallback)."""
+    X, fb = ldiv_solve(V, np.ascontiguousarray(M.T))
+    return np.ascontiguousarray(X.T), fb
+
+
+# ------------------------------------------------------------------ convergence
+def norm_factors(gram_list, lam) -> float:
+    """converge_checks.jl:5-11: (G_1 .* ... .* G_N) contracted with lambda lambda'."""
+    had = gram_list[0].copy()
+    for G in gram_list[1:]:
+        had = had * G
+    return float(lam @ had @ lam)
+
+
+def fit_from_parts(ref_norm, inner, fact_square) -> float:
+    """fit_check.jl:29-32: 1 - sqrt(|ref^2 + ||X||^2 - 2|<T,X>||) / ref."""
+    normResidual = math.sqrt(abs(ref_norm * ref_norm + fact_square - 2 * abs(inner)))
+    return 1.0 - normResidual / ref_norm
+
+
+@dataclass
+class NoCheck:
+    """no_check.jl:1-6 (lastfit starts at -1)."""
+    max_counter: int
+    iter: int = 0
+    lastfit: float = -1.0
+
+
+@dataclass
+class FitCheck:
+    """fit_check.jl:5-20: iter=0, counter=0, lastfit=1, final_fit=0, total_iter=0."""
+    tolerance: float
+    max_counter: int
+    ref_norm: float
+    iter: int = 0
+    counter: int = 0
+    MttKRP: Optional[np.ndarray] = None
+    lastfit: float = 1.0
+    final_fit: float = 0.0
+    total_iter: int = 0
+
+
+@dataclass
+class CPDiffCheck:
+    """cp_diff_check.jl:6-18."""
+    tolerance: float
+    max_counter: int
+    iter: int = 0
+    counter: int = 0
+    norm_prev_iter: float = 0.0
+    PrevCP: Optional[CPD] = None
+    lastfit: float = 1.0
+    final_fit: float = 0.0
+    total_iter: int = 0
+
+
+@dataclass
+class CPAngleCheck:
+    """cp_angle_check.jl:6-18."""
+    tolerance: float
+    max_counter: int
+    iter: int = 0
+    counter: int = 0
+    norm_prev_iter: float = 0.0
+    PrevCP: Optional[CPD] = None
+    lastangle: float = 1.0
+    final_fit: float = 0.0
+    total_iter: int = 0
+
+
+def CPDFit(check) -> float:
+    """fit_check.jl:68 / cp_diff_check.jl:73 / cp_angle_check.jl:75."""
+    return check.final_fit
+
+
+def cp_cp_contract(f1, f2) -> np.ndarray:
+    """cp_contract.jl:32-52: inner(r1, r2) = prod_n (A1_n^T A2_n)(r1, r2) (all modes shared)."""
+    inner = np.ones((f1[0].shape[1], f2[0].shape[1]))
+    for a, b in zip(f1, f2):
+        inner = inner * (a.T @ b)
+    return inner
+
+
+def check_converge(check, factors, lam, gram_list, mtkrp=None, verbose=False, rank=None) -> bool:
+    """MttkrpAlgorithm.jl:5-14 (save_mttkrp then the check) and the checks themselves."""
+    R = lam.shape[0] if rank is None else rank
+    if isinstance(check, NoCheck):                       # no_check.jl:8-20
+        check.iter += 1
+        if verbose:
+            print(f"{R}\t {check.iter}")
+        if check.iter == check.max_counter:
+            check.iter = 0
+            return True
+        return False
+    if isinstance(check, FitCheck):                      # fit_check.jl:22-66
+        check.MttKRP = mtkrp
+        inner = float(np.sum(check.MttKRP * (factors[-1] * lam[None, :])))
+        fact_square = norm_factors(gram_list, lam)
+        curr_fit = fit_from_parts(check.ref_norm, inner, fact_square)
+        return fit_state_update(check, curr_fit, R, verbose)
+    if isinstance(check, (CPDiffCheck, CPAngleCheck)):   # cp_diff_check.jl:20-71, cp_angle_check.jl:20-73
+        check.iter += 1
+        cur_grams = [f.T @ f for f in factors]
+        if check.PrevCP is None:
+            check.PrevCP = CPD([f.copy() for f in factors], lam.copy())
+            nf = norm_factors(cur_grams, lam)
+            check.norm_prev_iter = nf if isinstance(check, CPDiffCheck) else math.sqrt(nf)
+            return False
+        cross = float(check.PrevCP.lam @ cp_cp_contract(check.PrevCP.factors, factors) @ lam)
+        if isinstance(check, CPDiffCheck):
+            fact_square = norm_factors(cur_grams, lam)
+            normResidual = math.sqrt(abs(check.norm_prev_iter + fact_square - 2 * abs(cross)))
+            cur = 1.0 - normResidual / math.sqrt(abs(check.norm_prev_iter))
+            dval = abs(check.lastfit - cur)
+            check.lastfit = cur
+            check.norm_prev_iter = fact_square
+            attr = "lastfit"
+        else:
+            norm_cur = math.sqrt(norm_factors(cur_grams, lam))
+            theta = cross / (norm_cur * check.norm_prev_iter)
+            theta = 1.0 if theta > 1.0 else theta
+            cur = math.acos(theta)
+            dval = abs(check.lastangle - cur)
+            check.lastangle = cur
+            check.norm_prev_iter = norm_cur
+            attr = "lastangle"
+        check.PrevCP = CPD([f.copy() for f in factors], lam.copy())
+        if verbose:
+            print(f"{R}\t {check.iter} \t {cur} \t {dval}")
+        done = _two_small_deltas(check, dval, attr)
+        if done or check.iter == 0:
+            check.PrevCP = None
+        return done
+    raise TypeError(f"unsupported check {type(check)}")
+
+
+def fit_state_update(check: "FitCheck", curr_fit: float, R: int = 0, verbose=False) -> bool:
+    """fit_check.jl:25,33-66 given the sweep's fit: iter += 1, delta, print, NaN throw,
+    then the two-consecutive-small-deltas rule."""
+    check.iter += 1
+    dfit = abs(check.lastfit - curr_fit)
+    check.lastfit = curr_fit
+    if verbose:
+        print(f"{R}\t {check.iter} \t {curr_fit} \t {dfit}")
+    if math.isnan(curr_fit):
+        raise OracleNaN("Error NAN")
+    return _two_small_deltas(check, dfit, "lastfit")
+
+
+def _two_small_deltas(check, delta: float, attr: str) -> bool:
+    """fit_check.jl:43-63: stop after two consecutive deltas < tol; reset on stop or max."""
+    if delta < check.tolerance:
+        check.counter += 1
+        if check.counter >= 2:
+            check.total_iter = check.iter
+            check.iter = 0
+            check.counter = 0
+            check.final_fit = getattr(check, attr)
+            setattr(check, attr, 0.0)
+            return True
+    else:
+        check.counter = 0
+    if check.iter >= check.max_counter:
+        check.total_iter = check.iter
+        check.iter = 0
+        check.counter = 0
+        check.final_fit = getattr(check, attr)
+        setattr(check, attr, 0.0)
+    return False
+
+
+# -------------------------------------------------------------------- optimiser
+@dataclass
+class ALSState:
+    """The ``ALS`` struct's additional_items that matter here (als_optimizer.jl:5-10)."""
+    gram_list: list
+    fallbacks: int = 0
+    fits: list = field(default_factory=list)
+
+
+def optimize(T, cp: CPD, alg=None, check=None, verbose=False, trace: Optional[dict] = None,
+             form: Optional[str] = None) -> CPD:
+    """optimize.jl:6-35 with the Gram cache seeded as in standard/tensor.jl:3-14.
+
+    Per sweep and per mode n: V <- compute_krp, M <- matricize_tensor, X <- solve,
+    (A_n, lambda) <- row_norm(X), G_n <- A_n^T A_n; then the check once per sweep
+    with the LAST mode's MTTKRP.  The input CPD is not mutated (optimize.jl:10-11).
+    """
+    alg = KRPFreeNormal() if alg is None else alg
+    check = NoCheck(100) if check is None else check
+    if form is None:
+        form = "tree"
+    st = ALSState(grams(cp.factors))
+    it = check.iter
+    lam = cp.lam.copy()
+    factors = [f.copy() for f in cp.factors]
+    N = len(factors)
+    T = np.asarray(T, dtype=np.float64)
+    while it < check.max_counter:
+        mtkrp = None
+        for n in range(N):
+            V = compute_krp(st.gram_list, n)
+            mtkrp = mttkrp(T, factors, n, form)
+            X, fb = solve_ls_problem(V, mtkrp)
+            st.fallbacks += int(fb)
+            factors[n], lam = row_norm(X)
+            st.gram_list[n] = factors[n].T @ factors[n]
+        done = check_converge(check, factors, lam, st.gram_list, mtkrp, verbose)
+        if isinstance(check, FitCheck):
+            st.fits.append(check.lastfit if not done else check.final_fit)
+        if done:
+            break
+        it += 1
+    if trace is not None:
+        trace["fallbacks"] = st.fallbacks
+        trace["fits"] = st.fits
+        trace["grams"] = st.gram_list
+    return CPD(factors, lam)
+
+
+def als_optimize(target, cp: CPD, alg=None, check=None, maxiter=None, verbose=False,
+                 trace=None, form=None) -> CPD:
+    """als_optimizer.jl:15-25 + compute_als defaults :37-55 (KRPFreeNormal, NoCheck(100))."""
+    check = NoCheck(100 if maxiter is None else maxiter) if check is None else check
+    return optimize(target, cp, alg, check, verbose, trace, form)
+
+
+def decompose(A, rank, solver=None, rng=None, alg=None, check=None, maxiter=None,
+              verbose=False, **kw) -> CPD:
+    """decompose.jl:13-30: random_CPD, then als_optimize; a solver raises."""
+    cp = random_CPD(A, rank, rng)
+    if solver is None:
+        return als_optimize(A, cp, alg, check, maxiter, verbose, **kw)
+    raise RuntimeError("OptimizerError")
+
+
+def increase_cpd_rank(cp: CPD, new_rank: int, rng=None) -> CPD:
+    """decompose.jl:71-83: fresh random factors with the old columns copied in."""
+    rng = np.random.default_rng(3) if rng is None else rng
+    old = cp.rank
+    assert new_rank >= old
+    factors, lam = random_factors([f.shape[0] for f in cp.factors], new_rank, rng)
+    for o, f in zip(cp.factors, factors):
+        f[:, :old] = o
+    return CPD(factors, lam)
+
+
+def decompose_adaptive(A, epsilon, max_rank, rng=None, alg=None, check=None, maxiter=None,
+                       verbose=False, start_rank=1, rank_step=1, **kw) -> CPD:
+    """decompose.jl:32-69: grow the rank until 1 - fit < epsilon or max_rank is passed."""
+    current = start_rank
+    cp = random_CPD(A, start_rank, rng)
+    check = FitCheck(1e-3, 100, float(np.linalg.norm(A))) if check is None else check
+    while True:
+        cp = als_optimize(A, cp, alg, check, maxiter, verbose, **kw)
+        check.iter = 0
+        if 1.0 - CPDFit(check) < epsilon:
+            return cp
+        current += rank_step
+        if current > max_rank:
+            return cp
+        cp = increase_cpd_rank(cp, current, rng)
+
+
+def reconstruct(factors, lam) -> np.ndarray:
+    """reconstruct.jl:2-9: sum_r lambda_r a_1r o a_2r o ... o a_Nr."""
+    N = len(factors)
+    letters = "abcdefghij"
+    spec = ",".join(f"{letters[n]}z" for n in range(N)) + ",z->" + letters[:N]
+    return np.einsum(spec, *factors, lam, optimize=True)

@@ -1,0 +1,260 @@
+"""Pins the CPU oracle (oracle/) against brute force, closed forms, LAPACK-independent
+identities, known-answer vectors and the reference's own end-to-end bars."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import rng as ORNG
+from conftest import lowrank
+
+
+def brute_mttkrp(T, F, n):
+    R = F[0].shape[1]
+    M = np.zeros((T.shape[n], R))
+    for idx in itertools.product(*[range(d) for d in T.shape]):
+        for r in range(R):
+            w = T[idx]
+            for m, i in enumerate(idx):
+                if m != n:
+                    w *= F[m][i, r]
+            M[idx[n], r] += w
+    return M
+
+
+# ---------------------------------------------------------------- row_norm
+@pytest.mark.parametrize("shape", [(20, 30), (7, 1), (1, 5)])
+def test_row_norm_unit_columns(shape):
+    # basic_features.jl:1-16: every normalised column has unit 2-norm
+    A = np.random.default_rng(1).standard_normal(shape)
+    An, lam = O.row_norm(A)
+    np.testing.assert_allclose((An ** 2).sum(axis=0), 1.0, rtol=1e-14)
+    np.testing.assert_allclose(An * lam[None, :], A, rtol=1e-14)
+    np.testing.assert_allclose(lam, [math.sqrt(sum(x * x for x in A[:, r])) for r in range(shape[1])],
+                               rtol=1e-14)
+
+
+# ------------------------------------------------------------------ mttkrp
+@pytest.mark.parametrize("dims", [(4, 3, 5), (3, 4, 2, 3), (5, 6)])
+@pytest.mark.parametrize("form", ["krpfree", "krp", "tree"])
+def test_mttkrp_brute_force(dims, form):
+    rng = np.random.default_rng(len(dims))
+    T = rng.standard_normal(dims)
+    F = [rng.standard_normal((d, 3)) for d in dims]
+    for n in range(len(dims)):
+        np.testing.assert_allclose(O.mttkrp(T, F, n, form), brute_mttkrp(T, F, n), rtol=1e-12,
+                                   atol=1e-12)
+
+
+def test_mttkrp_rank1_closed_form():
+    # T = a o b o c  ->  M_1 = a (b^T B .* c^T C)
+    rng = np.random.default_rng(5)
+    a, b, c = rng.standard_normal(6), rng.standard_normal(5), rng.standard_normal(4)
+    T = np.einsum("i,j,k->ijk", a, b, c)
+    F = [rng.standard_normal((d, 3)) for d in (6, 5, 4)]
+    want = np.outer(a, (b @ F[1]) * (c @ F[2]))
+    for form in ("krpfree", "krp", "tree"):
+        np.testing.assert_allclose(O.mttkrp(T, F, 0, form), want, rtol=1e-12)
+
+
+def test_mttkrp_krp_gemm_identity():
+    # M_n = T_(n) (A_N (.) ... (.) A_1 without n), Julia column-major unfolding
+    rng = np.random.default_rng(6)
+    dims = (5, 4, 3, 2)
+    T = rng.standard_normal(dims)
+    F = [rng.standard_normal((d, 4)) for d in dims]
+    for n in range(4):
+        others = [m for m in range(4) if m != n]
+        Tn = np.moveaxis(T, n, 0).reshape(dims[n], -1, order="F")
+        K = np.ones((1, 4))
+        for m in others:           # first other mode fastest in the column index
+            K = (F[m][:, None, :] * K[None, :, :]).reshape(-1, 4)
+        np.testing.assert_allclose(O.mttkrp(T, F, n, "tree"), Tn @ K, rtol=1e-12)
+
+
+def test_compute_krp_and_grams():
+    rng = np.random.default_rng(7)
+    F = [rng.standard_normal((d, 5)) for d in (6, 7, 8)]
+    G = O.grams(F)
+    for g, f in zip(G, F):
+        np.testing.assert_allclose(g, [[sum(f[i, r] * f[i, s] for i in range(f.shape[0]))
+                                        for s in range(5)] for r in range(5)], rtol=1e-13)
+    V = O.compute_krp(G, 1)
+    np.testing.assert_allclose(V, G[0] * G[2], rtol=0, atol=0)
+
+
+# ------------------------------------------------------------------- solve
+def test_ldiv_full_rank_matches_solve():
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((9, 9))
+    V = A @ A.T + 9 * np.eye(9)
+    B = rng.standard_normal((9, 4))
+    X, fb = O.ldiv_solve(V, B)
+    assert not fb
+    np.testing.assert_allclose(X, np.linalg.solve(V, B), rtol=1e-12)
+
+
+def test_ldiv_rank_deficient_is_min_norm():
+    # exact rank 4 of 7 with a clear gap: the QR-CP complete orthogonal solve is pinv's
+    rng = np.random.default_rng(9)
+    Q, _ = np.linalg.qr(rng.standard_normal((7, 7)))
+    V = Q[:, :4] @ np.diag([5.0, 3.0, 2.0, 1.0]) @ Q[:, :4].T
+    B = rng.standard_normal((7, 3))
+    X, fb = O.ldiv_solve(V, B)
+    assert fb
+    np.testing.assert_allclose(X, np.linalg.pinv(V) @ B, rtol=1e-9, atol=1e-10)
+
+
+def test_ldiv_pivot_threshold_is_absolute_1e6():
+    # dpstrf stops when the next pivot is <= tol (absolute), which triggers the QR path
+    B = np.ones((2, 1))
+    _, fb_hi = O.ldiv_solve(np.diag([1.0, 2e-6]), B)
+    _, fb_lo = O.ldiv_solve(np.diag([1.0, 5e-7]), B)
+    assert not fb_hi and fb_lo
+    # the first pivot is only checked for > 0 (the largest diagonal)
+    _, fb_small = O.ldiv_solve(np.diag([5e-7]), np.ones((1, 1)))
+    assert not fb_small
+
+
+def test_ldiv_duplicate_columns_falls_back():
+    rng = np.random.default_rng(10)
+    A = rng.standard_normal((12, 5))
+    A[:, 3] = A[:, 1]
+    V = A.T @ A
+    B = rng.standard_normal((5, 6))
+    X, fb = O.ldiv_solve(V, B)
+    assert fb
+    np.testing.assert_allclose(X, np.linalg.pinv(V, rcond=1e-12) @ B, rtol=1e-8, atol=1e-8)
+
+
+# --------------------------------------------------------------------- fit
+def test_fit_formula_matches_explicit_residual():
+    T, _ = lowrank((10, 9, 8), 3, 0.1, 11)
+    cp = O.random_CPD(T, 3, np.random.default_rng(12))
+    chk = O.FitCheck(0.0, 1, float(np.linalg.norm(T)))
+    out = O.als_optimize(T, cp, check=chk)
+    explicit = 1 - np.linalg.norm(T - O.reconstruct(out.factors, out.lam)) / np.linalg.norm(T)
+    assert abs(chk.final_fit - explicit) < 1e-10
+
+
+def test_reconstruct_brute_force():
+    # basic_features.jl:60-92
+    rng = np.random.default_rng(13)
+    F = [rng.standard_normal((d, 4)) for d in (5, 6, 7)]
+    lam = rng.standard_normal(4)
+    want = np.zeros((5, 6, 7))
+    for r in range(4):
+        for i, j, k in itertools.product(range(5), range(6), range(7)):
+            want[i, j, k] += lam[r] * F[0][i, r] * F[1][j, r] * F[2][k, r]
+    np.testing.assert_allclose(O.reconstruct(F, lam), want, rtol=1e-13, atol=1e-13)
+
+
+# ---------------------------------------------------------- check machines
+def test_fitcheck_state_machine_scripted():
+    c = O.FitCheck(1e-3, 100, 1.0)
+    seq = [0.5, 0.9, 0.9005, 0.9007]
+    res = [O.fit_state_update(c, f) for f in seq]
+    assert res == [False, False, False, True]
+    assert (c.total_iter, c.final_fit, c.iter, c.counter, c.lastfit) == (4, 0.9007, 0, 0, 0.0)
+    # a large delta in between resets the counter
+    c = O.FitCheck(1e-3, 100, 1.0)
+    res = [O.fit_state_update(c, f) for f in [0.9, 0.9001, 0.95, 0.9501, 0.9502]]
+    assert res == [False, False, False, False, True] and c.total_iter == 5
+
+
+def test_fitcheck_max_iterations_resets_without_converging():
+    c = O.FitCheck(1e-9, 3, 1.0)
+    res = [O.fit_state_update(c, f) for f in [0.1, 0.2, 0.3]]
+    assert res == [False, False, False]
+    assert (c.total_iter, c.final_fit, c.iter, c.lastfit) == (3, 0.3, 0, 0.0)
+
+
+def test_fitcheck_nan_raises():
+    c = O.FitCheck(1e-3, 10, 1.0)
+    with pytest.raises(O.OracleNaN):
+        O.fit_state_update(c, float("nan"))
+
+
+def test_nocheck_runs_exactly_max_sweeps():
+    T, _ = lowrank((6, 5, 4), 2, 0.1, 14)
+    cp = O.random_CPD(T, 2, np.random.default_rng(1))
+    chk = O.NoCheck(7)
+    calls = []
+    orig = O.als.check_converge
+
+    def spy(*a, **k):
+        calls.append(1)
+        return orig(*a, **k)
+    O.als.check_converge = spy
+    try:
+        O.als_optimize(T, cp, check=chk)
+    finally:
+        O.als.check_converge = orig
+    assert len(calls) == 7 and chk.iter == 0
+
+
+def test_optimize_does_not_mutate_input():
+    T, _ = lowrank((6, 5, 4), 2, 0.1, 15)
+    cp = O.random_CPD(T, 2, np.random.default_rng(2))
+    before = cp.copy()
+    O.als_optimize(T, cp, check=O.NoCheck(3))
+    assert all(np.array_equal(a, b) for a, b in zip(cp.factors, before.factors))
+
+
+# ----------------------------------------------------------- Philox stream
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32 10 rounds
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, want in kat:
+        got = ORNG.philox4x32_10(*[[c] for c in ctr], *key)
+        assert tuple(int(g[0]) for g in got) == want
+
+
+def test_philox_normals_are_standard_normal():
+    z = ORNG.normal_matrix(400, 250, seed=3, mode=0)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+    f, lam = ORNG.random_factors_philox((30, 20), 4, 9)
+    np.testing.assert_allclose((f[0] ** 2).sum(0), 1.0, rtol=1e-14)
+    np.testing.assert_allclose(lam, np.linalg.norm(ORNG.normal_matrix(20, 4, 9, 1), axis=0), rtol=1e-14)
+
+
+# ------------------------------------------------ reference end-to-end bars
+@pytest.fixture(scope="module")
+def random_20_30_40():
+    return np.random.default_rng(100).standard_normal((20, 30, 40))
+
+
+def test_cp_als_bar_decompose_rank400(random_20_30_40):
+    # cp_als.jl:9-10: decompose(A, 400) (NoCheck(100)) reconstructs to < 1e-7
+    A = random_20_30_40
+    tr = {}
+    out = O.decompose(A, 400, trace=tr)
+    assert np.linalg.norm(O.reconstruct(out.factors, out.lam) - A) / np.linalg.norm(A) < 1e-7
+    # every V here keeps its pivots above 1e-6: this bar runs on the Cholesky path
+    assert tr["fallbacks"] == 0
+
+
+def test_cp_als_bar_fitcheck(random_20_30_40):
+    # cp_als.jl:17-19 and :33-34
+    A = random_20_30_40
+    chk = O.FitCheck(1e-6, 100, float(np.linalg.norm(A)))
+    out = O.decompose(A, 400, check=chk)
+    assert np.linalg.norm(A - O.reconstruct(out.factors, out.lam)) / np.linalg.norm(A) < 1e-5
+    cp = O.random_CPD(A, 400)
+    out = O.als_optimize(A, cp, alg=O.KRPNormal(), check=O.NoCheck(10), form="krp")
+    assert np.linalg.norm(O.reconstruct(out.factors, out.lam) - A) / np.linalg.norm(A) < 1e-1
+
+
+def test_cp_als_bar_exact_rank4():
+    # cp_als.jl:56-69: exact rank-4 target, R = 400, FitCheck(1e-6, 100)
+    rng = np.random.default_rng(101)
+    f, lam = O.random_factors((20, 30, 40), 4, rng)
+    A = O.reconstruct(f, lam)
+    chk = O.FitCheck(1e-6, 100, float(np.linalg.norm(A)))
+    out = O.decompose(A, 400, check=chk)
+    assert np.linalg.norm(O.reconstruct(out.factors, out.lam) - A) / np.linalg.norm(A) < 1e-3
