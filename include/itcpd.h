@@ -69,7 +69,10 @@ typedef struct {
 typedef struct itcpd_plan itcpd_plan;
 
 /* Per-plan timing of the MTTKRP GEMM kernels, measured with HIP events on the
- * plan's stream when profiling is on (itcpd_plan_set_profiling). */
+ * plan's stream when profiling is on (itcpd_plan_set_profiling).  The left GEMM
+ * (output rows contiguous in T) and the right GEMM (reduction contiguous in T)
+ * are kept apart; ITCPD_KERNEL_GEMM_ALL sums both. */
+enum { ITCPD_KERNEL_GEMM_LEFT = 0, ITCPD_KERNEL_GEMM_RIGHT = 1, ITCPD_KERNEL_GEMM_ALL = 2 };
 typedef struct {
     int64_t launches;      /* GEMM launches timed */
     double total_ms;       /* sum of their durations */
@@ -148,9 +151,9 @@ int itcpd_fit(itcpd_plan* plan, const double* M_last, int32_t mem, double ref_no
 /* norm(T) on the device (the ref_norm handed to FitCheck). */
 int itcpd_tensor_norm(itcpd_plan* plan, double* norm_out);
 
-/* ---- instrumentation ---- */
+/* ---- instrumentation (no counterpart in the reference) ---- */
 int itcpd_plan_set_profiling(itcpd_plan* plan, int32_t on);
-int itcpd_get_kernel_stats(itcpd_plan* plan, itcpd_kernel_stats* stats);
+int itcpd_get_kernel_stats(itcpd_plan* plan, int32_t kernel, itcpd_kernel_stats* stats);
 int itcpd_synchronize(itcpd_plan* plan);
 
 /* Thread-local text of the last failing call on this thread ("" if none). */

@@ -18,6 +18,7 @@ ITCPD_FP64, ITCPD_FP32, ITCPD_BF16X3 = 0, 1, 2
 ITCPD_ALG_KRP_FREE_NORMAL, ITCPD_ALG_KRP_NORMAL = 0, 1
 ITCPD_MEM_HOST, ITCPD_MEM_DEVICE = 0, 1
 ITCPD_CHECK_NONE, ITCPD_CHECK_FIT = 0, 1
+ITCPD_KERNEL_GEMM_LEFT, ITCPD_KERNEL_GEMM_RIGHT, ITCPD_KERNEL_GEMM_ALL = 0, 1, 2
 
 # Every function the header declares, with (restype, argtypes).
 _P = C.c_void_p
@@ -59,7 +60,7 @@ SIGNATURES = {
     "itcpd_fit": (C.c_int, [_P, _P, C.c_int32, C.c_double, _DP, _DP, _DP]),
     "itcpd_tensor_norm": (C.c_int, [_P, _DP]),
     "itcpd_plan_set_profiling": (C.c_int, [_P, C.c_int32]),
-    "itcpd_get_kernel_stats": (C.c_int, [_P, C.POINTER(KernelStats)]),
+    "itcpd_get_kernel_stats": (C.c_int, [_P, C.c_int32, C.POINTER(KernelStats)]),
     "itcpd_synchronize": (C.c_int, [_P]),
     "itcpd_last_error": (C.c_char_p, []),
     "itcpd_version": (C.c_char_p, []),

@@ -316,9 +316,10 @@ class ALSPlan:
     def set_profiling(self, on: bool):
         L.check(self._lib.itcpd_plan_set_profiling(self._h, 1 if on else 0))
 
-    def kernel_stats(self) -> dict:
+    def kernel_stats(self, kernel: int = L.ITCPD_KERNEL_GEMM_ALL) -> dict:
+        """HIP-event timing of the MTTKRP GEMM launches since profiling was switched on."""
         st = L.KernelStats()
-        L.check(self._lib.itcpd_get_kernel_stats(self._h, C.byref(st)))
+        L.check(self._lib.itcpd_get_kernel_stats(self._h, int(kernel), C.byref(st)))
         return {"launches": st.launches, "total_ms": st.total_ms, "flops": st.flops,
                 "bytes": st.bytes}
 

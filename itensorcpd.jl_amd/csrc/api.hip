@@ -100,23 +100,42 @@ int reset_check_flags(itcpd_plan* p) {
     return ITCPD_OK;
 }
 
-int collect_timer(itcpd_plan* p) {
+// Folds finished begin/end event pairs into the per-kernel totals.  With
+// wait == false only pairs whose end event has already completed are taken
+// (the sweep loop calls it that way so the host never stalls the queue).
+int collect_timer(itcpd_plan* p, bool wait) {
     KernelTimer& tm = p->timer;
     const size_t npair = tm.ev.size() / 2;
-    for (size_t k = 0; k < npair; ++k) {
+    size_t k = 0;
+    for (; k < npair; ++k) {
+        if (!wait) {
+            const hipError_t q = hipEventQuery(tm.ev[2 * k + 1]);
+            if (q == hipErrorNotReady) break;
+            ITCPD_HIP(q);
+        }
         float ms = 0.f;
         ITCPD_HIP(hipEventSynchronize(tm.ev[2 * k + 1]));
         ITCPD_HIP(hipEventElapsedTime(&ms, tm.ev[2 * k], tm.ev[2 * k + 1]));
-        tm.total_ms += ms;
-        tm.flops += tm.pending_flops[k];
-        tm.bytes += tm.pending_bytes[k];
-        tm.launches += 1;
+        KernelTimer::Acc& a = tm.acc[tm.pending_kind[k]];
+        a.total_ms += ms;
+        a.flops += tm.pending_flops[k];
+        a.bytes += tm.pending_bytes[k];
+        a.launches += 1;
+        (void)hipEventDestroy(tm.ev[2 * k]);
+        (void)hipEventDestroy(tm.ev[2 * k + 1]);
     }
-    for (hipEvent_t e : tm.ev) (void)hipEventDestroy(e);
-    tm.ev.clear();
-    tm.pending_flops.clear();
-    tm.pending_bytes.clear();
+    tm.ev.erase(tm.ev.begin(), tm.ev.begin() + 2 * k);
+    tm.pending_flops.erase(tm.pending_flops.begin(), tm.pending_flops.begin() + k);
+    tm.pending_bytes.erase(tm.pending_bytes.begin(), tm.pending_bytes.begin() + k);
+    tm.pending_kind.erase(tm.pending_kind.begin(), tm.pending_kind.begin() + k);
     return ITCPD_OK;
+}
+
+// Main and side streams, both at the default priority.
+hipError_t create_streams(itcpd_plan* p) {
+    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    return hipStreamCreateWithFlags(&p->side_stream, hipStreamNonBlocking);
 }
 
 // Per-mode factor state shared by plans and the plan-free random init.
@@ -134,31 +153,47 @@ int alloc_factor_state(itcpd_plan* p) {
     ITCPD_HIP(hipMemset(p->lam, 0, (size_t)Rp * 8));
     if ((rc = dmalloc(&p->X, (size_t)(maxI * Rp)))) return rc;
     if ((rc = dmalloc(&p->M, (size_t)(maxI * Rp)))) return rc;
+    if ((rc = dmalloc(&p->Msum, (size_t)(maxI * Rp)))) return rc;
+    if ((rc = dmalloc(&p->counters, 64))) return rc;
+    ITCPD_HIP(hipMemset(p->counters, 0, 64 * sizeof(unsigned)));
     if ((rc = dmalloc(&p->sumsq_part, (size_t)(128 * R)))) return rc;
     {
-        // Gram partials: at most 64 blocks and at most 2^25 doubles
-        int64_t nb = std::min<int64_t>(64, std::max<int64_t>(1, ((int64_t)1 << 25) / (R * R)));
+        // Gram partials: one per row block of the fused solve / normalisation kernels,
+        // at most 256 blocks and at most 2^25 doubles
+        int64_t nb = std::min<int64_t>(256, std::max<int64_t>(1, ((int64_t)1 << 25) / (R * R)));
         if ((rc = dmalloc(&p->gram_part, (size_t)(nb * R * R)))) return rc;
-        if ((rc = dmalloc(&p->inner_part, (size_t)nb))) return rc;
+        if ((rc = dmalloc(&p->inner_part, 256))) return rc;
     }
     if ((rc = dmalloc(&p->scal, 8))) return rc;
     return ITCPD_OK;
 }
 
+// One ALS sweep (optimize.jl:17-30).  Mode n: V_n is factored, M_n computed, then
+// solve, row_norm, Gram refresh (and the fit after the last mode) run on the main
+// stream.  The first mode of each side gets M from a GEMM launch that fills every
+// CU, so its factorisation runs just before that GEMM; the other modes get M from
+// a short Hadamard-reduce and their factorisation runs beside it on the side stream.
 int sweep(itcpd_plan* p) {
     int rc;
     const int N = p->N;
-    if ((rc = launch_left_gemm(p))) return rc;
+    MSrc ms{};
+    if ((rc = launch_factor(p, 0))) return rc;
+    if ((rc = launch_left_gemm(p, &ms))) return rc;
     for (int n = 0; n < p->s; ++n) {
-        if (p->s > 1 && (rc = launch_side_ttv(p, n))) return rc;
-        if ((rc = launch_update_mode(p, n, n == N - 1))) return rc;
+        if (n > 0 && (rc = fork_factor(p, n))) return rc;
+        if (p->s > 1 && (rc = launch_side_ttv(p, n, &ms))) return rc;
+        if (n > 0 && (rc = join_factor(p, n))) return rc;
+        if ((rc = launch_solve(p, n, ms.ptr, ms.nparts, ms.stride, true, n == N - 1))) return rc;
     }
-    if ((rc = launch_right_gemm(p))) return rc;
+    if ((rc = launch_factor(p, p->s))) return rc;
+    if ((rc = launch_right_gemm(p, &ms))) return rc;
     for (int n = p->s; n < N; ++n) {
-        if (N - p->s > 1 && (rc = launch_side_ttv(p, n))) return rc;
-        if ((rc = launch_update_mode(p, n, n == N - 1))) return rc;
+        if (n > p->s && (rc = fork_factor(p, n))) return rc;
+        if (N - p->s > 1 && (rc = launch_side_ttv(p, n, &ms))) return rc;
+        if (n > p->s && (rc = join_factor(p, n))) return rc;
+        if ((rc = launch_solve(p, n, ms.ptr, ms.nparts, ms.stride, true, n == N - 1))) return rc;
     }
-    if (p->profiling && p->timer.ev.size() >= 64) return collect_timer(p);
+    if (p->profiling && p->timer.ev.size() >= 64) return collect_timer(p, false);
     return ITCPD_OK;
 }
 
@@ -221,8 +256,15 @@ int itcpd_plan_create(itcpd_plan** out, int32_t order, const int64_t* dims, int6
         itcpd_plan_destroy(p);
         return code;
     };
-    hipError_t e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    hipError_t e = create_streams(p);
     if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+    for (int q = 0; q < order; ++q) {
+        e = hipEventCreateWithFlags(&p->ev_main[q], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_side[q], hipEventDisableTiming);
+        if (e == hipSuccess && q < ITCPD_POLL_SLOTS)
+            e = hipEventCreateWithFlags(&p->ev_poll[q], hipEventDisableTiming);
+        if (e != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
+    }
 
     const int64_t R = p->R, Rp = p->Rp, maxI = p->maxI;
     const size_t nel = (size_t)(p->ML * p->MR);
@@ -239,6 +281,7 @@ int itcpd_plan_create(itcpd_plan** out, int32_t order, const int64_t* dims, int6
     }
     if ((rc = alloc_factor_state(p))) return bail(rc);
     if ((rc = dmalloc(&p->U, (size_t)(R * R)))) return bail(rc);
+    if ((rc = dmalloc(&p->invdiag, (size_t)R))) return bail(rc);
     if ((rc = dmalloc(&p->qr_aux, (size_t)(R * R + 4 * R)))) return bail(rc);
     if ((rc = dmalloc(&p->piv, (size_t)(2 * R)))) return bail(rc);
     if ((rc = dmalloc(&p->finfo, 4))) return bail(rc);
@@ -252,9 +295,10 @@ int itcpd_plan_create(itcpd_plan** out, int32_t order, const int64_t* dims, int6
         if ((rc = dmalloc(&p->parts, want))) return bail(rc);
     }
     if ((rc = dmalloc(&p->chk, 1))) return bail(rc);
-    e = hipHostMalloc((void**)&p->chk_host, sizeof(DevCheck), hipHostMallocDefault);
+    e = hipHostMalloc((void**)&p->chk_host, sizeof(DevCheck) * (1 + ITCPD_POLL_SLOTS),
+                      hipHostMallocDefault);
     if (e != hipSuccess) return bail(hip_fail(e, "hipHostMalloc"));
-    std::memset(p->chk_host, 0, sizeof(DevCheck));
+    std::memset(p->chk_host, 0, sizeof(DevCheck) * (1 + ITCPD_POLL_SLOTS));
     e = hipMemcpy(p->chk, p->chk_host, sizeof(DevCheck), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy(check)"));
     p->trace_cap = 1024;
@@ -280,6 +324,9 @@ int itcpd_plan_destroy(itcpd_plan* p) {
     dfree(p->X);
     dfree(p->M);
     dfree(p->U);
+    dfree(p->invdiag);
+    dfree(p->Msum);
+    dfree(p->counters);
     dfree(p->qr_aux);
     dfree(p->piv);
     dfree(p->finfo);
@@ -293,6 +340,13 @@ int itcpd_plan_destroy(itcpd_plan* p) {
     dfree(p->chk);
     dfree(p->fit_trace);
     if (p->chk_host) (void)hipHostFree(p->chk_host);
+    if (p->side_stream) (void)hipStreamSynchronize(p->side_stream);
+    for (int q = 0; q < ITCPD_MAX_ORDER; ++q) {
+        if (p->ev_main[q]) (void)hipEventDestroy(p->ev_main[q]);
+        if (p->ev_side[q]) (void)hipEventDestroy(p->ev_side[q]);
+        if (q < ITCPD_POLL_SLOTS && p->ev_poll[q]) (void)hipEventDestroy(p->ev_poll[q]);
+    }
+    if (p->side_stream) (void)hipStreamDestroy(p->side_stream);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return ITCPD_OK;
@@ -411,22 +465,45 @@ int itcpd_run(itcpd_plan* p, itcpd_check* c, double* fit_trace) {
     ITCPD_HIP(hipMemcpyAsync(p->chk, h, sizeof(DevCheck), hipMemcpyHostToDevice, p->stream));
     ITCPD_HIP(hipStreamSynchronize(p->stream));
 
-    const int64_t batch = (c->kind == ITCPD_CHECK_FIT) ? 4 : nsweeps;
+    // FitCheck: every POLL_EVERY sweeps a snapshot of the device check state is
+    // copied to a pinned slot behind an event.  The host keeps enqueueing and
+    // only waits once two snapshots are outstanding, so the queue never drains;
+    // sweeps enqueued after convergence return at their first instruction (each
+    // kernel tests chk->done).
+    constexpr int64_t POLL_EVERY = 4;
+    const int64_t batch = (c->kind == ITCPD_CHECK_FIT) ? POLL_EVERY : nsweeps;
+    DevCheck* slots = p->chk_host + 1;
+    int64_t polls_issued = 0, polls_seen = 0;
     int rc = ITCPD_OK;
     for (int64_t k = 0; k < nsweeps;) {
         const int64_t nb = std::min<int64_t>(batch, nsweeps - k);
         for (int64_t b = 0; b < nb; ++b)
             if ((rc = sweep(p))) return rc;
         k += nb;
-        if (c->kind == ITCPD_CHECK_FIT && k < nsweeps) {
-            ITCPD_HIP(hipMemcpyAsync(h, p->chk, sizeof(DevCheck), hipMemcpyDeviceToHost, p->stream));
-            ITCPD_HIP(hipStreamSynchronize(p->stream));
-            if (h->done) break;
+        if (c->kind != ITCPD_CHECK_FIT || k >= nsweeps) continue;
+        const int slot = (int)(polls_issued % ITCPD_POLL_SLOTS);
+        ITCPD_HIP(hipMemcpyAsync(&slots[slot], p->chk, sizeof(DevCheck), hipMemcpyDeviceToHost,
+                                 p->stream));
+        ITCPD_HIP(hipEventRecord(p->ev_poll[slot], p->stream));
+        ++polls_issued;
+        bool stop = false;
+        while (polls_seen < polls_issued) {
+            const int os = (int)(polls_seen % ITCPD_POLL_SLOTS);
+            if (polls_issued - polls_seen >= 2) {
+                ITCPD_HIP(hipEventSynchronize(p->ev_poll[os]));
+            } else {
+                const hipError_t q = hipEventQuery(p->ev_poll[os]);
+                if (q == hipErrorNotReady) break;
+                ITCPD_HIP(q);
+            }
+            ++polls_seen;
+            if (slots[os].done) { stop = true; break; }
         }
+        if (stop) break;
     }
     ITCPD_HIP(hipMemcpyAsync(h, p->chk, sizeof(DevCheck), hipMemcpyDeviceToHost, p->stream));
     ITCPD_HIP(hipStreamSynchronize(p->stream));
-    if (p->profiling && (rc = collect_timer(p))) return rc;
+    if (p->profiling && (rc = collect_timer(p, true))) return rc;
     c->iter = h->iter;
     c->counter = h->counter;
     c->total_iter = h->total_iter;
@@ -478,7 +555,7 @@ int itcpd_solve_ls(itcpd_plan* p, int32_t mode, const double* M, double* X_out, 
     const size_t bytes = (size_t)p->dims[mode] * p->R * 8;
     if ((rc = copy_in(p->M, M, bytes, mem, p->stream))) return rc;
     if ((rc = launch_factor(p, mode))) return rc;
-    if ((rc = launch_solve_rows(p, mode, p->M))) return rc;
+    if ((rc = launch_solve(p, mode, p->M, 1, (int64_t)p->dims[mode] * p->R, false, false))) return rc;
     if ((rc = copy_out(X_out, p->X, bytes, mem, p->stream))) return rc;
     int info[4];
     ITCPD_HIP(hipMemcpyAsync(info, p->finfo, sizeof(info), hipMemcpyDeviceToHost, p->stream));
@@ -534,20 +611,30 @@ int itcpd_tensor_norm(itcpd_plan* p, double* norm_out) {
 
 int itcpd_plan_set_profiling(itcpd_plan* p, int32_t on) {
     if (!p) return fail(ITCPD_E_ARG, "null plan");
+    DeviceGuard dg(p->device);
+    int rc;
+    if ((rc = collect_timer(p, true))) return rc;   // retire outstanding events
     p->profiling = on != 0;
     p->timer = KernelTimer();
     return ITCPD_OK;
 }
 
-int itcpd_get_kernel_stats(itcpd_plan* p, itcpd_kernel_stats* st) {
+int itcpd_get_kernel_stats(itcpd_plan* p, int32_t kernel, itcpd_kernel_stats* st) {
     if (!p || !st) return fail(ITCPD_E_ARG, "null argument");
+    if (kernel < ITCPD_KERNEL_GEMM_LEFT || kernel > ITCPD_KERNEL_GEMM_ALL)
+        return fail(ITCPD_E_ARG, "unknown kernel id");
     DeviceGuard dg(p->device);
     int rc;
-    if ((rc = collect_timer(p))) return rc;
-    st->launches = p->timer.launches;
-    st->total_ms = p->timer.total_ms;
-    st->flops = p->timer.flops;
-    st->bytes = p->timer.bytes;
+    if ((rc = collect_timer(p, true))) return rc;
+    *st = itcpd_kernel_stats{0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < 2; ++k) {
+        if (kernel != ITCPD_KERNEL_GEMM_ALL && kernel != k) continue;
+        const KernelTimer::Acc& a = p->timer.acc[k];
+        st->launches += a.launches;
+        st->total_ms += a.total_ms;
+        st->flops += a.flops;
+        st->bytes += a.bytes;
+    }
     return ITCPD_OK;
 }
 

@@ -11,6 +11,7 @@
 
 #define ITCPD_MAX_ORDER 8
 #define ITCPD_MAX_RANK 1024
+#define ITCPD_POLL_SLOTS 4       // in-flight FitCheck state snapshots per run
 
 namespace itcpd {
 
@@ -49,10 +50,14 @@ struct Side {
 };
 
 struct KernelTimer {
+    struct Acc {
+        int64_t launches = 0;
+        double total_ms = 0, flops = 0, bytes = 0;
+    };
     std::vector<hipEvent_t> ev;          // begin/end pairs recorded this run
-    int64_t launches = 0;
-    double total_ms = 0, flops = 0, bytes = 0;
+    Acc acc[2];                          // 0: left GEMM, 1: right GEMM
     std::vector<double> pending_flops, pending_bytes;
+    std::vector<int> pending_kind;
 };
 
 }  // namespace itcpd
@@ -60,6 +65,9 @@ struct KernelTimer {
 struct itcpd_plan {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side_stream = nullptr;         // the R x R factorisations run here
+    hipEvent_t ev_main[ITCPD_MAX_ORDER] = {nullptr};
+    hipEvent_t ev_side[ITCPD_MAX_ORDER] = {nullptr};
     int N = 0;
     int64_t dims[ITCPD_MAX_ORDER] = {0};
     int64_t R = 0, Rp = 0;   // rank and padded rank (multiple of 16 * RB)
@@ -83,6 +91,9 @@ struct itcpd_plan {
     double* M = nullptr;                       // maxI x R column-major (summed MTTKRP)
     double* U = nullptr;                       // R x R factor workspace (Cholesky / QR)
     double* qr_aux = nullptr;                  // QR fallback workspace (tau, Z reflectors)
+    double* invdiag = nullptr;                 // R: 1 / U(j,j) of the pivoted Cholesky
+    double* Msum = nullptr;                    // maxI x R: summed MTTKRP of the last mode (fit)
+    unsigned* counters = nullptr;              // arrival counters of last-block reductions
     int* piv = nullptr;                        // R
     int* finfo = nullptr;                      // [0] = rank, [1] = used fallback, [2] = QR rank
     double* PL = nullptr;                      // ML x Rp partial of the left side (if s > 1)
@@ -96,12 +107,20 @@ struct itcpd_plan {
     itcpd::DevCheck* chk = nullptr;
     double* fit_trace = nullptr;               // device, length trace_cap
     int trace_cap = 0;
-    itcpd::DevCheck* chk_host = nullptr;       // pinned mirror
+    itcpd::DevCheck* chk_host = nullptr;       // pinned mirror, then ITCPD_POLL_SLOTS poll slots
+    hipEvent_t ev_poll[ITCPD_POLL_SLOTS] = {nullptr};
     bool profiling = false;
     itcpd::KernelTimer timer;
 };
 
 namespace itcpd {
+
+// An MTTKRP result: M(i, r) = sum_{z < nparts} ptr[z * stride + i + I_n * r].
+struct MSrc {
+    const double* ptr;
+    int nparts;
+    int64_t stride;
+};
 
 // error plumbing (api.hip)
 void set_error(const std::string& msg);
@@ -109,16 +128,22 @@ int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
 // launchers (mttkrp.hip)
-int launch_left_gemm(itcpd_plan* p);      // PL (or M of mode 0 when s == 1)
-int launch_right_gemm(itcpd_plan* p);     // PR (or M of mode N-1 when N-s == 1)
-int launch_side_ttv(itcpd_plan* p, int mode);  // M of `mode` from PL / PR
+int launch_left_gemm(itcpd_plan* p, MSrc* ms);   // PL (or M of mode 0 when s == 1)
+int launch_right_gemm(itcpd_plan* p, MSrc* ms);  // PR (or M of mode N-1 when N-s == 1)
+int launch_side_ttv(itcpd_plan* p, int mode, MSrc* ms);  // M of `mode` from PL / PR
 int mttkrp_mode(itcpd_plan* p, int mode);      // full M_mode into p->M from the current factors
+int launch_sum_parts(itcpd_plan* p, const MSrc& ms, int64_t n, double* out);
 int launch_tensor_norm(itcpd_plan* p);
 
 // launchers (solve.hip)
-int launch_update_mode(itcpd_plan* p, int mode, bool fit_after);  // solve + row_norm + post_solve
+int launch_update_mode(itcpd_plan* p, int mode, bool fit_after, const double* Msrc, int nparts,
+                       int64_t stride);  // factor + solve + row_norm + post_solve (+ fit)
+int fork_factor(itcpd_plan* p, int mode);  // factor of `mode` on the side stream
+int join_factor(itcpd_plan* p, int mode);  // main stream waits for it
 int launch_factor(itcpd_plan* p, int mode);
-int launch_solve_rows(itcpd_plan* p, int mode, const double* Msrc);
+int launch_factor_on(itcpd_plan* p, int mode, hipStream_t st);
+int launch_solve(itcpd_plan* p, int mode, const double* Msrc, int nparts, int64_t stride,
+                 bool finalize, bool fit_after);
 int launch_normalize(itcpd_plan* p, int mode, const double* Xsrc, bool fit_after);
 int launch_seed_grams(itcpd_plan* p);
 int launch_random_init(itcpd_plan* p, uint64_t seed);

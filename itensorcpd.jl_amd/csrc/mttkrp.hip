@@ -161,8 +161,11 @@ __global__ __launch_bounds__(256, 2) void k_tk(const TT* __restrict__ T, int64_t
 // ---------------------------------------------------------------- TR GEMM
 // out(mR, r) = sum over mL = i + I0*rest of T(mL, mR) KRP(mL, r),
 // KRP(mL, r) = A0[i + I0*r] * prod_{q>=1} Aq[digit_q + Iq*r] (column-major factors).
-// Wave tile: 16*MB output rows x 16*RB ranks.  Over a group of 4*S reduction
-// indices, lane (g, c) owns the run i0 + S*g .. + S-1 (contiguous in T and in A0).
+// Workgroup tile: 16*MB output rows x 16*RB ranks; the 4 waves share the tile and
+// split the workgroup's reduction range [kk0, kk1) (in groups of 4*S indices of
+// the first left mode) into four consecutive parts, then add their accumulators
+// through LDS.  Over a group, lane (g, c) owns the run i0 + S*g .. + S-1
+// (contiguous in T and in A0).
 template <typename TT, int RB, int MB, int S, bool FAST>
 __global__ __launch_bounds__(256, 2) void k_tr(const TT* __restrict__ T, int64_t ML, int64_t MR,
                                                Side sd, int64_t ngroups_i, int64_t kk_per_split,
@@ -170,13 +173,16 @@ __global__ __launch_bounds__(256, 2) void k_tr(const TT* __restrict__ T, int64_t
                                                double* __restrict__ out, int64_t split_stride,
                                                const DevCheck* __restrict__ chk) {
     if (chk->done) return;
+    __shared__ d4 red[2][RB * MB][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
-    const int64_t n0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * MB);
-    if (n0 >= MR) return;
+    const int64_t n0 = (int64_t)blockIdx.x * (16 * MB);
     const int r0 = blockIdx.y * (16 * RB);
-    const int64_t kk0 = (int64_t)blockIdx.z * kk_per_split;
-    const int64_t kk1 = kk0 + kk_per_split < kk_total ? kk0 + kk_per_split : kk_total;
+    const int64_t kb0 = (int64_t)blockIdx.z * kk_per_split;
+    const int64_t kb1 = kb0 + kk_per_split < kk_total ? kb0 + kk_per_split : kk_total;
+    const int64_t quarter = (kb1 - kb0 + 3) / 4;
+    const int64_t kk0 = kb0 + wave * quarter < kb1 ? kb0 + wave * quarter : kb1;
+    const int64_t kk1 = kk0 + quarter < kb1 ? kk0 + quarter : kb1;
     const int64_t I0 = sd.dim[0];
     const double* __restrict__ A0 = sd.fac[0];
 
@@ -263,6 +269,34 @@ __global__ __launch_bounds__(256, 2) void k_tr(const TT* __restrict__ T, int64_t
         kk = rest * ngroups_i + s_hi;
     }
 
+    // combine the four waves' partial sums: (2,3) -> (0,1), then 1 -> 0
+    if (wave >= 2) {
+#pragma unroll
+        for (int a = 0; a < RB; ++a)
+#pragma unroll
+            for (int b = 0; b < MB; ++b) red[wave - 2][a * MB + b][lane] = acc[a][b];
+    }
+    __syncthreads();
+    if (wave < 2) {
+#pragma unroll
+        for (int a = 0; a < RB; ++a)
+#pragma unroll
+            for (int b = 0; b < MB; ++b) acc[a][b] += red[wave][a * MB + b][lane];
+    }
+    __syncthreads();
+    if (wave == 1) {
+#pragma unroll
+        for (int a = 0; a < RB; ++a)
+#pragma unroll
+            for (int b = 0; b < MB; ++b) red[0][a * MB + b][lane] = acc[a][b];
+    }
+    __syncthreads();
+    if (wave != 0) return;
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int b = 0; b < MB; ++b) acc[a][b] += red[0][a * MB + b][lane];
+
     double* o = out + (int64_t)blockIdx.z * split_stride;
 #pragma unroll
     for (int a = 0; a < RB; ++a)
@@ -318,7 +352,18 @@ __global__ __launch_bounds__(256) void k_ttv_lead(const double* __restrict__ P, 
     double sum = 0.0;
     if (sd.nm - pos - 1 == 1) {
         const double* __restrict__ f = sd.fac[pos + 1] + sd.dim[pos + 1] * r;
-        for (int64_t b = b0; b < b1; ++b) sum += Pc[In * b] * f[b];
+        int64_t b = b0;
+        for (; b + 8 <= b1; b += 8) {
+            double pv[8], fv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                pv[k] = Pc[In * (b + k)];
+                fv[k] = f[b + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) sum += pv[k] * fv[k];
+        }
+        for (; b < b1; ++b) sum += Pc[In * b] * f[b];
     } else {
         for (int64_t b = b0; b < b1; ++b) sum += Pc[In * b] * w_after(sd, pos, b, r);
     }
@@ -344,7 +389,18 @@ __global__ __launch_bounds__(256) void k_ttv_inner(const double* __restrict__ P,
         const double* __restrict__ Pc = P + MS * r + A * i + A * In * b;
         double part = 0.0;
         if (one_before) {
-            for (int64_t a = lane; a < A; a += 64) part += Pc[a] * f0[a];
+            int64_t a = lane;
+            for (; a + 64 * 7 < A; a += 64 * 8) {
+                double pv[8], fv[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    pv[k] = Pc[a + 64 * k];
+                    fv[k] = f0[a + 64 * k];
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) part += pv[k] * fv[k];
+            }
+            for (; a < A; a += 64) part += Pc[a] * f0[a];
         } else {
             for (int64_t a = lane; a < A; a += 64) part += Pc[a] * w_before(sd, pos, a, r);
         }
@@ -364,7 +420,15 @@ __global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ pa
     const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (x >= n) return;
     double s = 0.0;
-    for (int z = 0; z < nz; ++z) s += parts[(int64_t)z * stride + x];
+    int z = 0;
+    for (; z + 8 <= nz; z += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = parts[(int64_t)(z + k) * stride + x];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += v[k];
+    }
+    for (; z < nz; ++z) s += parts[(int64_t)z * stride + x];
     out[x] = s;
 }
 
@@ -374,8 +438,16 @@ __global__ __launch_bounds__(256) void k_sumsq_part(const TT* __restrict__ T, in
                                                     double* __restrict__ part) {
     __shared__ double red[256];
     double s = 0.0;
-    for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < n;
-         x += (int64_t)gridDim.x * 256) {
+    const int64_t st = (int64_t)gridDim.x * 256;
+    int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    for (; x + 7 * st < n; x += 8 * st) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = (double)T[x + k * st];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += v[k] * v[k];
+    }
+    for (; x < n; x += st) {
         const double v = (double)T[x];
         s += v * v;
     }
@@ -404,7 +476,7 @@ constexpr int TR_RUN = 2;
 struct Timed {
     itcpd_plan* p;
     hipEvent_t e1 = nullptr;
-    Timed(itcpd_plan* pp, double flops, double bytes) : p(pp) {
+    Timed(itcpd_plan* pp, int kind, double flops, double bytes) : p(pp) {
         if (!p->profiling) return;
         hipEvent_t e0;
         if (hipEventCreate(&e0) != hipSuccess) return;
@@ -413,6 +485,7 @@ struct Timed {
         p->timer.ev.push_back(e0);
         p->timer.pending_flops.push_back(flops);
         p->timer.pending_bytes.push_back(bytes);
+        p->timer.pending_kind.push_back(kind);
     }
     ~Timed() {
         if (!e1) return;
@@ -461,8 +534,8 @@ template <typename TT, int RB, int MB>
 int run_tr(itcpd_plan* p, const Side& sd, double* out, int64_t nz, int64_t kk_per,
            int64_t kk_total, int64_t ngroups, int64_t split_stride) {
     constexpr int S = TR_RUN;
-    dim3 grid((unsigned)cdiv(p->MR, 64 * MB), (unsigned)(p->Rp / (16 * RB)), (unsigned)nz);
-    const bool fast = (p->MR % (64 * MB) == 0) && (sd.dim[0] % (4 * S) == 0) && (p->ML % 2 == 0);
+    dim3 grid((unsigned)cdiv(p->MR, 16 * MB), (unsigned)(p->Rp / (16 * RB)), (unsigned)nz);
+    const bool fast = (p->MR % (16 * MB) == 0) && (sd.dim[0] % (4 * S) == 0) && (p->ML % 2 == 0);
     if (fast)
         hipLaunchKernelGGL((k_tr<TT, RB, MB, S, true>), grid, dim3(256), 0, p->stream,
                            (const TT*)p->T, p->ML, p->MR, sd, ngroups, kk_per, kk_total,
@@ -498,18 +571,39 @@ int dispatch_tr(itcpd_plan* p, const Side& sd, double* out, int64_t nz, int64_t 
 
 int mb_of(int RB) { return RB <= 2 ? 8 : RB == 4 ? 4 : 2; }
 
-int sum_parts(itcpd_plan* p, const double* parts, int nz, int64_t stride, int64_t n, double* out) {
-    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, p->stream, parts,
-                       nz, stride, n, out, p->chk);
+}  // namespace
+
+// Parts with at most this many partials are summed by the consumer (the solve).
+static constexpr int MAX_CONSUMER_PARTS = 16;
+
+int launch_sum_parts(itcpd_plan* p, const MSrc& ms, int64_t n, double* out) {
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, p->stream, ms.ptr,
+                       ms.nparts, ms.stride, n, out, p->chk);
     ITCPD_HIP(hipGetLastError());
     return ITCPD_OK;
 }
 
+namespace {
+// Hands split partials to the consumer, or sums them into dst when there are many.
+int finish_parts(itcpd_plan* p, double* parts, int64_t nz, int64_t n, double* dst, MSrc* ms,
+                 bool consumer_sums) {
+    if (nz == 1) {
+        *ms = MSrc{dst, 1, n};
+        return ITCPD_OK;
+    }
+    MSrc pm{parts, (int)nz, n};
+    if (consumer_sums && nz <= MAX_CONSUMER_PARTS) {
+        *ms = pm;
+        return ITCPD_OK;
+    }
+    *ms = MSrc{dst, 1, n};
+    return launch_sum_parts(p, pm, n, dst);
+}
 }  // namespace
 
-// Left GEMM: rows = left multi-index (ML), reduction = right side.  Writes PL
-// (s > 1) or the M of mode 0 into p->M (s == 1).
-int launch_left_gemm(itcpd_plan* p) {
+// Left GEMM: rows = left multi-index (ML), reduction = right side.  Produces PL
+// (s > 1) or the M of mode 0 (s == 1).
+int launch_left_gemm(itcpd_plan* p, MSrc* ms) {
     const Side sd = make_side(p, p->s, p->N - p->s, /*rank_fast=*/true);
     const int MB = mb_of(p->RB);
     const int64_t nsteps_i = cdiv(sd.dim[0], 4);
@@ -526,28 +620,27 @@ int launch_left_gemm(itcpd_plan* p) {
     double* out = nz > 1 ? p->parts : dst;
     int rc;
     {
-        Timed tm(p, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
+        Timed tm(p, 0, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
                  (double)p->ML * (double)p->MR * p->elt_bytes);
         rc = (p->elt_bytes == 8)
                  ? dispatch_tk<double>(p, sd, out, p->ML, nz, kk_per, kk_total, nsteps_i, out_elems)
                  : dispatch_tk<float>(p, sd, out, p->ML, nz, kk_per, kk_total, nsteps_i, out_elems);
     }
     if (rc) return rc;
-    if (nz > 1) return sum_parts(p, p->parts, (int)nz, out_elems, out_elems, dst);
-    return ITCPD_OK;
+    return finish_parts(p, p->parts, nz, out_elems, dst, ms, p->s == 1);
 }
 
-// Right GEMM: rows = right multi-index (MR), reduction = left side.  Writes PR
-// (N-s > 1) or the M of mode N-1 into p->M (N-s == 1).
-int launch_right_gemm(itcpd_plan* p) {
+// Right GEMM: rows = right multi-index (MR), reduction = left side.  Produces PR
+// (N-s > 1) or the M of mode N-1 (N-s == 1).
+int launch_right_gemm(itcpd_plan* p, MSrc* ms) {
     const Side sd = make_side(p, 0, p->s, /*rank_fast=*/false);
     const int MB = mb_of(p->RB);
     constexpr int S = TR_RUN;
     const int64_t ngroups = cdiv(sd.dim[0], 4 * S);
     const int64_t nrest = p->ML / sd.dim[0];
     const int64_t kk_total = ngroups * nrest;
-    const int64_t base = cdiv(p->MR, 64 * MB) * (p->Rp / (16 * p->RB));
-    int64_t nz = pick_split(base, kk_total, 512, 8);
+    const int64_t base = cdiv(p->MR, 16 * MB) * (p->Rp / (16 * p->RB));
+    int64_t nz = pick_split(base, kk_total, 512, 32);
     const int64_t out_elems = p->MR * p->R;
     if (nz > 1 && (size_t)(nz * out_elems) > p->parts_elems) nz = (int64_t)(p->parts_elems / out_elems);
     if (nz < 1) nz = 1;
@@ -557,19 +650,18 @@ int launch_right_gemm(itcpd_plan* p) {
     double* out = nz > 1 ? p->parts : dst;
     int rc;
     {
-        Timed tm(p, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
+        Timed tm(p, 1, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
                  (double)p->ML * (double)p->MR * p->elt_bytes);
         rc = (p->elt_bytes == 8)
                  ? dispatch_tr<double>(p, sd, out, nz, kk_per, kk_total, ngroups, out_elems)
                  : dispatch_tr<float>(p, sd, out, nz, kk_per, kk_total, ngroups, out_elems);
     }
     if (rc) return rc;
-    if (nz > 1) return sum_parts(p, p->parts, (int)nz, out_elems, out_elems, dst);
-    return ITCPD_OK;
+    return finish_parts(p, p->parts, nz, out_elems, dst, ms, p->N - p->s == 1);
 }
 
 // M of `mode` from its side's partial (only for sides with more than one mode).
-int launch_side_ttv(itcpd_plan* p, int mode) {
+int launch_side_ttv(itcpd_plan* p, int mode, MSrc* ms) {
     const bool left = mode < p->s;
     const int first = left ? 0 : p->s;
     const int nm = left ? p->s : p->N - p->s;
@@ -585,7 +677,8 @@ int launch_side_ttv(itcpd_plan* p, int mode) {
     if (A == 1) {
         int64_t nz = 1;
         const int64_t blocks = cdiv(nout, 256);
-        while (blocks * nz < 2048 && B / (nz * 2) >= 16 && (size_t)(nz * 2 * nout) <= p->parts_elems)
+        while (blocks * nz < 2048 && B / (nz * 2) >= 16 && (size_t)(nz * 2 * nout) <= p->parts_elems &&
+               nz * 2 <= MAX_CONSUMER_PARTS)
             nz *= 2;
         const int64_t bper = cdiv(B, nz);
         nz = cdiv(B, bper);
@@ -593,25 +686,28 @@ int launch_side_ttv(itcpd_plan* p, int mode) {
         hipLaunchKernelGGL(k_ttv_lead, dim3((unsigned)blocks, (unsigned)nz), dim3(256), 0, p->stream,
                            P, MS, In, (int)p->R, sd, pos, B, bper, out, nout, p->chk);
         ITCPD_HIP(hipGetLastError());
-        if (nz > 1) return sum_parts(p, p->parts, (int)nz, nout, nout, p->M);
-        return ITCPD_OK;
+        return finish_parts(p, p->parts, nz, nout, p->M, ms, true);
     }
     hipLaunchKernelGGL(k_ttv_inner, dim3((unsigned)cdiv(nout, 4)), dim3(256), 0, p->stream, P, MS,
                        In, (int)p->R, sd, pos, A, B, p->M, p->chk);
     ITCPD_HIP(hipGetLastError());
+    *ms = MSrc{p->M, 1, nout};
     return ITCPD_OK;
 }
 
-// Full MTTKRP of one mode from the current factors, into p->M (hook entry point).
+// Full MTTKRP of one mode from the current factors, summed into p->M (hook).
 int mttkrp_mode(itcpd_plan* p, int mode) {
     int rc;
+    MSrc ms{};
+    const int64_t n = p->dims[mode] * p->R;
     if (mode < p->s) {
-        if ((rc = launch_left_gemm(p))) return rc;
-        if (p->s > 1) return launch_side_ttv(p, mode);
-        return ITCPD_OK;
+        if ((rc = launch_left_gemm(p, &ms))) return rc;
+        if (p->s > 1 && (rc = launch_side_ttv(p, mode, &ms))) return rc;
+    } else {
+        if ((rc = launch_right_gemm(p, &ms))) return rc;
+        if (p->N - p->s > 1 && (rc = launch_side_ttv(p, mode, &ms))) return rc;
     }
-    if ((rc = launch_right_gemm(p))) return rc;
-    if (p->N - p->s > 1) return launch_side_ttv(p, mode);
+    if (ms.nparts > 1) return launch_sum_parts(p, ms, n, p->M);
     return ITCPD_OK;
 }
 

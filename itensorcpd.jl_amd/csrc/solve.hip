@@ -25,7 +25,110 @@ struct GramSet {
     int N;
 };
 
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
 static constexpr int FACTOR_THREADS = 512;
+
+
+// Wave-wide argmax of (v, i): largest v, smallest i on ties; NaN never wins.
+// DPP row_ror 8/4/2/1 all-reduces each row of 16 lanes, then the four row
+// results are combined from readlane values (uniform result in every lane).
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp_step(double& v, int& i) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int olo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+    const int ohi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    const int oi = __builtin_amdgcn_update_dpp(0, i, CTRL, 0xF, 0xF, false);
+    const double ov = __hiloint2double(ohi, olo);
+    if (ov > v || (ov == v && oi < i)) {
+        v = ov;
+        i = oi;
+    }
+}
+__device__ __forceinline__ void wave_argmax(double& v, int& i) {
+    argmax_dpp_step<0x128>(v, i);   // row_ror:8
+    argmax_dpp_step<0x124>(v, i);   // row_ror:4
+    argmax_dpp_step<0x122>(v, i);   // row_ror:2
+    argmax_dpp_step<0x121>(v, i);   // row_ror:1
+    double bv = v;
+    int bi = i;
+    bv = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 0),
+                          __builtin_amdgcn_readlane(__double2loint(v), 0));
+    bi = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+    for (int r = 1; r < 4; ++r) {
+        const double rv = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 16 * r),
+                                           __builtin_amdgcn_readlane(__double2loint(v), 16 * r));
+        const int ri = __builtin_amdgcn_readlane(i, 16 * r);
+        if (rv > bv || (rv == bv && ri < bi)) {
+            bv = rv;
+            bi = ri;
+        }
+    }
+    v = bv;
+    i = bi;
+}
+
+// ----------------------------------------------------- batched global loads
+// Small kernels here are latency bound: every helper issues all of its loads
+// before the first use, so a thread waits on memory once per batch.
+
+// (1 .* G_a .* G_b ...) over the modes != skip, in mode order, at element x
+// (MttkrpAlgorithm.jl:38-47; with skip < 0 it is norm_factors' product).
+__device__ __forceinline__ double had_at(const GramSet& gs, int skip, int64_t x) {
+    double g[ITCPD_MAX_ORDER];
+#pragma unroll
+    for (int m = 0; m < ITCPD_MAX_ORDER; ++m)
+        g[m] = (m < gs.N && m != skip) ? gs.G[m][x] : 1.0;
+    double h = 1.0;
+#pragma unroll
+    for (int m = 0; m < ITCPD_MAX_ORDER; ++m) h *= g[m];
+    return h;
+}
+
+// sum_{b < n} p[b * stride + x] in increasing b, eight loads in flight.
+__device__ __forceinline__ double sum_parts_at(const double* __restrict__ p, int64_t stride, int n,
+                                               int64_t x) {
+    double acc = 0.0;
+    int b = 0;
+    for (; b + 8 <= n; b += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = p[(int64_t)(b + k) * stride + x];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k];
+    }
+    for (; b < n; ++b) acc += p[(int64_t)b * stride + x];
+    return acc;
+}
+
+// dst[x] = (1 .* G_a .* G_b ...)(x) over the modes != skip, in mode order, for
+// x < n: per mode, 16 element loads per thread are in flight together.
+template <int NT>
+__device__ __forceinline__ void fill_hadamard(double* dst, const GramSet& gs, int skip, int n) {
+    for (int x0 = threadIdx.x; x0 < n; x0 += NT * 16) {
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = 1.0;
+        for (int m = 0; m < gs.N; ++m) {
+            if (m == skip) continue;
+            const double* __restrict__ G = gs.G[m];
+            double g[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int x = x0 + NT * k;
+                g[k] = G[x < n ? x : 0];
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] *= g[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int x = x0 + NT * k;
+            if (x < n) dst[x] = v[k];
+        }
+    }
+}
 
 // --------------------------------------------------------- block reductions
 template <int NT>
@@ -74,6 +177,129 @@ __device__ int block_argmax(int lo, int hi, F f, double* redv, int* redi) {
     const int r = redi[0];
     __syncthreads();
     return r;
+}
+
+template <int RQ>
+__device__ __forceinline__ double bcast(const double (&y)[RQ], int j) {
+    const int q = j >> 6, ln = j & 63;
+    double v = 0.0;
+#pragma unroll
+    for (int qq = 0; qq < RQ; ++qq)
+        if (qq == q) v = y[qq];
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), ln);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), ln);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// QR path of the row solve (column-pivoted QR minimum-norm data in W):
+// y <- Q^T y, T11 solve on the leading rk entries, y <- Z^T y.
+template <int RQ>
+__device__ void fallback_apply(double (&y)[RQ], int lane, int R, int rk, const double* __restrict__ W,
+                               const double* __restrict__ qtau, const double* __restrict__ ztau) {
+    if (rk == 0) {
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) y[q] = 0.0;
+        return;
+    }
+    for (int tq = 0; tq < R; ++tq) {
+        const double tau = qtau[tq];
+        if (tau == 0.0) continue;
+        double part = 0.0;
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int k = lane + 64 * q;
+            if (k == tq) part += y[q];
+            else if (k > tq && k < R) part += W[k + (int64_t)R * tq] * y[q];
+        }
+        const double f = tau * wave_sum(part);
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int k = lane + 64 * q;
+            if (k == tq) y[q] -= f;
+            else if (k > tq && k < R) y[q] -= f * W[k + (int64_t)R * tq];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RQ; ++q)
+        if (lane + 64 * q >= rk) y[q] = 0.0;
+    for (int j = rk - 1; j >= 0; --j) {
+        const double wj = bcast<RQ>(y, j) / W[j + (int64_t)R * j];
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int k = lane + 64 * q;
+            if (k == j) y[q] = wj;
+            else if (k < j) y[q] -= W[k + (int64_t)R * j] * wj;
+        }
+    }
+    if (rk < R) {
+        for (int tq = 0; tq < rk; ++tq) {
+            const double tau = ztau[tq];
+            if (tau == 0.0) continue;
+            double part = 0.0;
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int k = lane + 64 * q;
+                if (k == tq) part += y[q];
+                else if (k >= rk && k < R) part += W[tq + (int64_t)R * k] * y[q];
+            }
+            const double f = tau * wave_sum(part);
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int k = lane + 64 * q;
+                if (k == tq) y[q] -= f;
+                else if (k >= rk && k < R) y[q] -= f * W[tq + (int64_t)R * k];
+            }
+        }
+    }
+}
+
+// The FitCheck / NoCheck update after a sweep (fit_check.jl:24-66, no_check.jl:8-20);
+// one thread.  fit = 1 - sqrt(|ref^2 + ||X||^2 - 2|<T,X>||) / ref.
+__device__ void check_update(DevCheck* chk, double inner, double fact_square, double* scal,
+                             double* trace, int trace_cap) {
+    const double ref = chk->ref_norm;
+    const double normResidual = sqrt(fabs(ref * ref + fact_square - 2.0 * fabs(inner)));
+    const double fit = 1.0 - normResidual / ref;
+    scal[0] = inner;
+    scal[1] = fact_square;
+    scal[2] = fit;
+    if (chk->kind != ITCPD_CHECK_FIT) return;
+    chk->iter += 1;
+    const double dfit = fabs(chk->lastfit - fit);
+    chk->lastfit = fit;
+    if (trace && chk->sweeps < trace_cap) trace[chk->sweeps] = fit;
+    chk->sweeps += 1;
+    if (fit != fit) {
+        chk->nan_flag = 1;
+        chk->done = 1;
+        return;
+    }
+    if (dfit < chk->tolerance) {
+        chk->counter += 1;
+        if (chk->counter >= 2) {
+            chk->total_iter = chk->iter;
+            chk->iter = 0;
+            chk->counter = 0;
+            chk->final_fit = chk->lastfit;
+            chk->lastfit = 0.0;
+            chk->done = 1;
+            return;
+        }
+    } else {
+        chk->counter = 0;
+    }
+    if (chk->iter >= chk->max_counter) {
+        chk->total_iter = chk->iter;
+        chk->iter = 0;
+        chk->counter = 0;
+        chk->final_fit = chk->lastfit;
+        chk->lastfit = 0.0;
+    }
 }
 
 // ------------------------------------------------------ xLAIC1 restatement
@@ -190,103 +416,19 @@ __device__ void laic1(int job, double alpha, double sest, double gamma, double* 
     *s = sine / tmp; *c = cosine / tmp;
 }
 
-// ------------------------------------------------------------ factorisation
-// One workgroup.  Builds V = Hadamard of the Grams of modes != mode, factors it
-// with pivoted Cholesky (upper, right-looking, pivot = largest remaining diagonal,
-// first index on ties; stop when the pivot is <= tol, or <= 0 / NaN at the first
-// step).  Full rank: U (row-major, ld R) and piv are written, finfo[1] = 0.
-// Otherwise the column-pivoted QR solve data are prepared in W (column-major):
-// Householder vectors below the diagonal, T11 and the RZ vectors above it, with
-// qtau, ztau, jpvt, and finfo[2] = numerical rank; finfo[1] = 1.
-__global__ __launch_bounds__(FACTOR_THREADS) void k_factor(GramSet gs, int mode, int R, double tol,
-                                                           double* __restrict__ U,
-                                                           int* __restrict__ piv,
-                                                           int* __restrict__ finfo,
-                                                           double* __restrict__ W,
-                                                           double* __restrict__ qtau,
-                                                           double* __restrict__ ztau,
-                                                           int* __restrict__ jpvt, double* vwork,
-                                                           int use_lds,
-                                                           const DevCheck* __restrict__ chk) {
-    if (chk->done) return;
-    extern __shared__ double lds[];
-    __shared__ double redv[FACTOR_THREADS];
-    __shared__ int redi[FACTOR_THREADS];
+// Column-pivoted QR minimum-norm solve data for V (rank-deficient case), all NT
+// threads of the block: W = V (column-major) overwritten by the xGEQP3 reflectors,
+// rank by xLAIC1 incremental condition estimation at rcond = R*eps, then the
+// xTZRZF reduction of the trapezoid.  finfo = {chol_rank, 1, qr_rank}.
+template <int NT>
+__device__ void qr_fallback(const GramSet& gs, int mode, int R, double* __restrict__ W,
+                            double* __restrict__ qtau, double* __restrict__ ztau,
+                            int* __restrict__ jpvt, double* vwork, int* __restrict__ finfo,
+                            double* redv, int* redi, int rank) {
     const int t = threadIdx.x;
-    constexpr int NT = FACTOR_THREADS;
-    double* S = use_lds ? lds : U;  // full symmetric R x R, row-major
-    int* pv = piv;
-
-    // V = ones .* G_m for m != mode, in mode order (MttkrpAlgorithm.jl:38-47)
-    for (int x = t; x < R * R; x += NT) {
-        double v = 1.0;
-        for (int m = 0; m < gs.N; ++m)
-            if (m != mode) v *= gs.G[m][x];
-        S[x] = v;
-    }
-    for (int i = t; i < R; i += NT) pv[i] = i;
-    __syncthreads();
-
-    int rank = R;
-    for (int j = 0; j < R; ++j) {
-        const int p = block_argmax<NT>(j, R, [&](int i) { return S[i * R + i]; }, redv, redi);
-        const double d = (p < R) ? S[p * R + p] : __builtin_nan("");
-        const bool bad = (j == 0) ? !(d > 0.0) : !(d > tol);
-        if (bad) { rank = j; break; }
-        if (p != j) {
-            // symmetric swap of indices j and p in the trailing block, and of
-            // columns j and p in the finished rows 0..j-1 of U
-            for (int k = t; k < R; k += NT) {
-                if (k >= j) {
-                    const double a = S[j * R + k];
-                    S[j * R + k] = S[p * R + k];
-                    S[p * R + k] = a;
-                }
-            }
-            __syncthreads();
-            for (int k = t; k < R; k += NT) {
-                const double a = S[k * R + j];
-                S[k * R + j] = S[k * R + p];
-                S[k * R + p] = a;
-            }
-            if (t == 0) {
-                const int q = pv[j];
-                pv[j] = pv[p];
-                pv[p] = q;
-            }
-            __syncthreads();
-        }
-        const double ujj = sqrt(S[j * R + j]);
-        const double inv = 1.0 / ujj;
-        __syncthreads();
-        if (t == 0) S[j * R + j] = ujj;
-        for (int k = j + 1 + t; k < R; k += NT) S[j * R + k] *= inv;
-        __syncthreads();
-        // trailing update of both triangles: S(k,l) -= U(j,k) U(j,l)
-        const int n = R - j - 1;
-        for (int x = t; x < n * n; x += NT) {
-            const int k = j + 1 + x / n, l = j + 1 + x % n;
-            S[k * R + l] -= S[j * R + k] * S[j * R + l];
-        }
-        __syncthreads();
-    }
-
-    if (rank == R) {
-        if (use_lds)
-            for (int x = t; x < R * R; x += NT) U[x] = S[x];
-        if (t == 0) { finfo[0] = R; finfo[1] = 0; finfo[2] = R; }
-        return;
-    }
-
-    // ---------------- fallback: column-pivoted QR minimum-norm solve data ----
     if (t == 0) { finfo[0] = rank; finfo[1] = 1; }
-    // W = V (column-major), recomputed from the Grams
-    for (int x = t; x < R * R; x += NT) {
-        double v = 1.0;
-        for (int m = 0; m < gs.N; ++m)
-            if (m != mode) v *= gs.G[m][x];
-        W[(x % R) + R * (x / R)] = v;   // V symmetric: same element either way
-    }
+    // W = V (column-major; V is symmetric), recomputed from the Grams
+    fill_hadamard<NT>(W, gs, mode, R * R);
     for (int i = t; i < R; i += NT) jpvt[i] = i;
     __syncthreads();
     double* vn1 = vwork;
@@ -453,172 +595,453 @@ __global__ __launch_bounds__(FACTOR_THREADS) void k_factor(GramSet gs, int mode,
     }
 }
 
-// --------------------------------------------------------------- row solves
-// X(i, :) = V^{-1} M(i, :) for every row i, one wave per row.  Lane l holds the
-// entries k = l + 64 q (q < RQ).  Also writes per-block partial column sums of
-// squares of X for row_norm.
-template <int RQ>
-__device__ __forceinline__ double bcast(const double (&y)[RQ], int j) {
-    const int q = j >> 6, ln = j & 63;
-    double v = 0.0;
-#pragma unroll
-    for (int qq = 0; qq < RQ; ++qq)
-        if (qq == q) v = y[qq];
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), ln);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), ln);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
 
-template <int RQ>
-__global__ __launch_bounds__(256) void k_solve_rows(const double* __restrict__ M, int64_t In, int R,
-                                                    const double* __restrict__ Ug,
-                                                    const int* __restrict__ piv,
-                                                    const int* __restrict__ finfo,
-                                                    const double* __restrict__ W,
-                                                    const double* __restrict__ qtau,
-                                                    const double* __restrict__ ztau,
-                                                    const int* __restrict__ jpvt, double* __restrict__ X,
-                                                    double* __restrict__ sumsq_part, int u_in_lds,
-                                                    const DevCheck* __restrict__ chk) {
+// ------------------------------------------------------------ factorisation
+// One workgroup.  Builds V = Hadamard of the Grams of modes != mode, factors it
+// with pivoted Cholesky (upper, right-looking, pivot = largest remaining diagonal,
+// first index on ties; stop when the pivot is <= tol, or <= 0 / NaN at the first
+// step).  Full rank: U (row-major, ld R) and piv are written, finfo[1] = 0.
+// Otherwise the column-pivoted QR solve data are prepared in W (column-major):
+// Householder vectors below the diagonal, T11 and the RZ vectors above it, with
+// qtau, ztau, jpvt, and finfo[2] = numerical rank; finfo[1] = 1.
+__global__ __launch_bounds__(FACTOR_THREADS) void k_factor(GramSet gs, int mode, int R, double tol,
+                                                           double* __restrict__ U,
+                                                           double* __restrict__ invdiag,
+                                                           int* __restrict__ piv,
+                                                           int* __restrict__ finfo,
+                                                           double* __restrict__ W,
+                                                           double* __restrict__ qtau,
+                                                           double* __restrict__ ztau,
+                                                           int* __restrict__ jpvt, double* vwork,
+                                                           int use_lds,
+                                                           const DevCheck* __restrict__ chk) {
     if (chk->done) return;
     extern __shared__ double lds[];
-    __shared__ double red[4][64 * RQ];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const bool fallback = finfo[1] != 0;
-    const int ldu = u_in_lds ? R + 1 : R;
-    const double* U = Ug;
-    if (!fallback && u_in_lds) {
-        for (int x = threadIdx.x; x < R * R; x += 256) lds[(x / R) * (R + 1) + (x % R)] = Ug[x];
+    __shared__ double redv[FACTOR_THREADS];
+    __shared__ int redi[FACTOR_THREADS];
+    const int t = threadIdx.x;
+    constexpr int NT = FACTOR_THREADS;
+    double* S = use_lds ? lds : U;  // full symmetric R x R, row-major
+    int* pv = piv;
+
+    // V = ones .* G_m for m != mode, in mode order (MttkrpAlgorithm.jl:38-47)
+    fill_hadamard<NT>(S, gs, mode, R * R);
+    for (int i = t; i < R; i += NT) pv[i] = i;
+    __syncthreads();
+
+    int rank = R;
+    for (int j = 0; j < R; ++j) {
+        const int p = block_argmax<NT>(j, R, [&](int i) { return S[i * R + i]; }, redv, redi);
+        const double d = (p < R) ? S[p * R + p] : __builtin_nan("");
+        const bool bad = (j == 0) ? !(d > 0.0) : !(d > tol);
+        if (bad) { rank = j; break; }
+        if (p != j) {
+            // symmetric swap of indices j and p in the trailing block, and of
+            // columns j and p in the finished rows 0..j-1 of U
+            for (int k = t; k < R; k += NT) {
+                if (k >= j) {
+                    const double a = S[j * R + k];
+                    S[j * R + k] = S[p * R + k];
+                    S[p * R + k] = a;
+                }
+            }
+            __syncthreads();
+            for (int k = t; k < R; k += NT) {
+                const double a = S[k * R + j];
+                S[k * R + j] = S[k * R + p];
+                S[k * R + p] = a;
+            }
+            if (t == 0) {
+                const int q = pv[j];
+                pv[j] = pv[p];
+                pv[p] = q;
+            }
+            __syncthreads();
+        }
+        const double ujj = sqrt(S[j * R + j]);
+        const double inv = 1.0 / ujj;
         __syncthreads();
-        U = lds;
+        if (t == 0) S[j * R + j] = ujj;
+        for (int k = j + 1 + t; k < R; k += NT) S[j * R + k] *= inv;
+        __syncthreads();
+        // trailing update of both triangles: S(k,l) -= U(j,k) U(j,l)
+        const int n = R - j - 1;
+        for (int x = t; x < n * n; x += NT) {
+            const int k = j + 1 + x / n, l = j + 1 + x % n;
+            S[k * R + l] -= S[j * R + k] * S[j * R + l];
+        }
+        __syncthreads();
     }
+
+    if (rank == R) {
+        // U = the factor (upper triangle of S; strict lower triangle zeroed)
+        for (int x = t; x < R * R; x += NT) {
+            const double v = (x % R) >= (x / R) ? S[x] : 0.0;
+            U[x] = v;
+        }
+        for (int j = t; j < R; j += NT) invdiag[j] = 1.0 / S[j * R + j];
+        if (t == 0) { finfo[0] = R; finfo[1] = 0; finfo[2] = R; }
+        return;
+    }
+    qr_fallback<NT>(gs, mode, R, W, qtau, ztau, jpvt, vwork, finfo, redv, redi, rank);
+}
+
+// Pivoted Cholesky for R <= 64 in one wave, no barriers in the pivot loop.
+// dpstf2 left-looking form on the unpermuted matrix V held in LDS: at step j the
+// pivot p is the remaining index with the largest V(p,p) - sum_t U(t,p)^2 (first
+// index on ties); row j of U is u_l = (V(p,l) - sum_{t<j} U(t,p) U(t,l)) / U(j,j)
+// for the remaining l.  Output in pivoted order: Uperm(j, t) = u^(j)_{piv[t]},
+// invdiag(j) = 1/U(j,j).  On rank deficiency all four waves build the QR data.
+// U is kept transposed, UbT(l, t) = U(t, l) with ld 66, so the
+// dot product sum_{t<j} U(t,p) U(t,l) reads row p (uniform: broadcast) and row l
+// of UbT with 16-byte loads at constant offsets, 16 terms per batch.
+__global__ __launch_bounds__(256) void k_factor64(GramSet gs, int mode, int R, double tol,
+                                                  double* __restrict__ Uperm,
+                                                  double* __restrict__ invdiag,
+                                                  int* __restrict__ piv, int* __restrict__ finfo,
+                                                  double* __restrict__ W, double* __restrict__ qtau,
+                                                  double* __restrict__ ztau, int* __restrict__ jpvt,
+                                                  double* vwork, const DevCheck* __restrict__ chk) {
+    if (chk && chk->done) return;
+    constexpr int LDT = 66;
+    extern __shared__ double S[];   // V: R x R | UbT: 64 x LDT
+    __shared__ double diagU[64];
+    __shared__ int pivS[64];
+    __shared__ int s_rank;
+    __shared__ double redv[256];
+    __shared__ int redi[256];
+    double* UbT = S + ((R * R + 1) & ~1);
+    const int t = threadIdx.x;
+    fill_hadamard<256>(S, gs, mode, R * R);
+    __syncthreads();
+    if (t < 64) {
+        const int l = t;
+        bool rem = l < R;
+        const double vdiag = rem ? S[l * R + l] : 0.0;
+        double sumsq = 0.0;
+        int rank = R;
+        for (int j = 0; j < R; ++j) {
+            double best = rem ? vdiag - sumsq : -__builtin_inf();
+            int bi = rem ? l : R;
+            wave_argmax(best, bi);
+            const int p = __builtin_amdgcn_readfirstlane(bi);
+            const double d = best;
+            const bool bad = (p >= R) || ((j == 0) ? !(d > 0.0) : !(d > tol));
+            if (bad) { rank = j; break; }
+            const double ujj = sqrt(d);
+            const double inv = 1.0 / ujj;
+            if (l == 0) { diagU[j] = ujj; pivS[j] = p; }
+            if (l == p) rem = false;
+            double u = 0.0;
+            if (rem) {
+                const double* __restrict__ up = UbT + p * LDT;
+                const double* __restrict__ ul = UbT + l * LDT;
+                double acc[4] = {0.0, 0.0, 0.0, 0.0};
+                int tt = 0;
+                for (; tt + 16 <= j; tt += 16) {
+                    dv2 a[8], b[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        a[k] = *reinterpret_cast<const dv2*>(up + tt + 2 * k);
+                        b[k] = *reinterpret_cast<const dv2*>(ul + tt + 2 * k);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        acc[k & 3] += a[k].x * b[k].x;
+                        acc[k & 3] += a[k].y * b[k].y;
+                    }
+                }
+                for (; tt < j; ++tt) acc[0] += up[tt] * ul[tt];
+                u = (S[p * R + l] - ((acc[0] + acc[1]) + (acc[2] + acc[3]))) * inv;
+                sumsq += u * u;
+            }
+            if (l < R) UbT[l * LDT + j] = u;   // zero for finished indices
+        }
+        if (l == 0) s_rank = rank;
+    }
+    __syncthreads();
+    const int rank = s_rank;
+    if (rank == R) {
+        for (int x = t; x < R * R; x += 256) {
+            const int j = x / R, tt = x % R;
+            double v = 0.0;
+            if (tt > j) v = UbT[pivS[tt] * LDT + j];
+            else if (tt == j) v = diagU[j];
+            Uperm[x] = v;
+        }
+        for (int j = t; j < R; j += 256) {
+            invdiag[j] = 1.0 / diagU[j];
+            piv[j] = pivS[j];
+        }
+        if (t == 0) { finfo[0] = R; finfo[1] = 0; finfo[2] = R; }
+        return;
+    }
+    qr_fallback<256>(gs, mode, R, W, qtau, ztau, jpvt, vwork, finfo, redv, redi, rank);
+}
+
+// Row solves and Gram tiles (no cross-block communication inside the kernel):
+// block b takes row tiles of rpb = 4 * RPW rows; each wave solves RPW rows at once
+// (their dependency chains interleave), x_i = V^{-1} m_i through U^T z = P^T m_i,
+// U w = z, x = P w (xPOTRS on the pivoted factor, reciprocal diagonal); m_i is kept
+// in Msum when given.
+// X is written to global memory and to an LDS tile whose X^T X is accumulated
+// into gram_part[b].  The QR fallback applies Q^T, T11^{-1}, Z^T row by row.
+template <int RQ, int RPW>
+__global__ __launch_bounds__(256) void k_solve(
+    const double* __restrict__ Msrc, int64_t In, int R,
+    const double* __restrict__ Ug, const double* __restrict__ invdiag, const int* __restrict__ piv,
+    const int* __restrict__ finfo, const double* __restrict__ W, const double* __restrict__ qtau,
+    const double* __restrict__ ztau, const int* __restrict__ jpvt, int u_in_lds,
+    double* __restrict__ X, double* __restrict__ Msum, double* __restrict__ gram_part, int grams,
+    const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    constexpr int RPB = 4 * RPW;
+    extern __shared__ double lds[];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const bool fallback = finfo[1] != 0;
+    double* Xs = lds;                                   // RPB x R tile
+    double* Us = lds + (size_t)RPB * R;                 // R x (R+1) when u_in_lds
+    const int ldu = R + 1;
+    const double* U = Ug;
+    int uld = R;
+    if (!fallback && u_in_lds) {
+        for (int x0 = t; x0 < R * R; x0 += 256 * 16) {
+            double v[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = (x0 + 256 * k < R * R) ? Ug[x0 + 256 * k] : 0.0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int x = x0 + 256 * k;
+                if (x < R * R) Us[(x / R) * ldu + (x % R)] = v[k];
+            }
+        }
+        U = Us;
+        uld = ldu;
+    }
+    __syncthreads();
     int pk[RQ];
+    double ivd[RQ];
 #pragma unroll
     for (int q = 0; q < RQ; ++q) {
         const int k = lane + 64 * q;
         pk[q] = (k < R) ? (fallback ? jpvt[k] : piv[k]) : 0;
+        ivd[q] = (k < R && !fallback) ? invdiag[k] : 0.0;
     }
     const int rk = fallback ? finfo[2] : R;
-    double ss[RQ];
+    double* gp = gram_part + (int64_t)blockIdx.x * R * R;
+    bool first_tile = true;
+    for (int64_t i0 = (int64_t)blockIdx.x * RPB; i0 < In; i0 += (int64_t)gridDim.x * RPB) {
+        const int nr = (int)((In - i0) < RPB ? (In - i0) : RPB);
+        const int rbase = wave * RPW;
+        double y[RPW][RQ];
 #pragma unroll
-    for (int q = 0; q < RQ; ++q) ss[q] = 0.0;
-
-    for (int64_t i = (int64_t)blockIdx.x * 4 + wave; i < In; i += (int64_t)gridDim.x * 4) {
-        double y[RQ];
-        if (!fallback) {
+        for (int rr = 0; rr < RPW; ++rr)
 #pragma unroll
             for (int q = 0; q < RQ; ++q) {
                 const int k = lane + 64 * q;
-                y[q] = (k < R) ? M[i + In * pk[q]] : 0.0;
+                const bool ok = rbase + rr < nr && k < R;
+                const int64_t off = ok ? i0 + rbase + rr + In * (fallback ? k : pk[q]) : 0;
+                y[rr][q] = Msrc[off];
             }
-            // U^T z = y (forward), then U w = z (backward)
-            for (int j = 0; j < R; ++j) {
-                const double zj = bcast<RQ>(y, j) / U[j * ldu + j];
+        if (Msum) {
+#pragma unroll
+            for (int rr = 0; rr < RPW; ++rr)
 #pragma unroll
                 for (int q = 0; q < RQ; ++q) {
                     const int k = lane + 64 * q;
-                    if (k == j) y[q] = zj;
-                    else if (k > j && k < R) y[q] -= U[j * ldu + k] * zj;
+                    if (rbase + rr < nr && k < R)
+                        Msum[i0 + rbase + rr + In * (fallback ? k : pk[q])] = y[rr][q];
+                }
+        }
+        if (!fallback) {
+            // forward: U^T z = y (rows interleaved)
+            for (int j = 0; j < R; ++j) {
+                double uj[RQ];
+#pragma unroll
+                for (int q = 0; q < RQ; ++q) {
+                    const int k = lane + 64 * q;
+                    uj[q] = (k > j && k < R) ? U[j * uld + k] : 0.0;
+                }
+                const double ij = bcast<RQ>(ivd, j);
+#pragma unroll
+                for (int rr = 0; rr < RPW; ++rr) {
+                    const double zj = bcast<RQ>(y[rr], j) * ij;
+#pragma unroll
+                    for (int q = 0; q < RQ; ++q) {
+                        const int k = lane + 64 * q;
+                        if (k == j) y[rr][q] = zj;
+                        else y[rr][q] -= uj[q] * zj;
+                    }
                 }
             }
+            // backward: U w = z
             for (int j = R - 1; j >= 0; --j) {
-                const double wj = bcast<RQ>(y, j) / U[j * ldu + j];
+                double uj[RQ];
 #pragma unroll
                 for (int q = 0; q < RQ; ++q) {
                     const int k = lane + 64 * q;
-                    if (k == j) y[q] = wj;
-                    else if (k < j) y[q] -= U[k * ldu + j] * wj;
+                    uj[q] = (k < j) ? U[k * uld + j] : 0.0;
+                }
+                const double ij = bcast<RQ>(ivd, j);
+#pragma unroll
+                for (int rr = 0; rr < RPW; ++rr) {
+                    const double wj = bcast<RQ>(y[rr], j) * ij;
+#pragma unroll
+                    for (int q = 0; q < RQ; ++q) {
+                        const int k = lane + 64 * q;
+                        if (k == j) y[rr][q] = wj;
+                        else y[rr][q] -= uj[q] * wj;
+                    }
                 }
             }
         } else {
 #pragma unroll
+            for (int rr = 0; rr < RPW; ++rr) fallback_apply<RQ>(y[rr], lane, R, rk, W, qtau, ztau);
+        }
+#pragma unroll
+        for (int rr = 0; rr < RPW; ++rr) {
+            if (rbase + rr >= nr) continue;
+            const int64_t i = i0 + rbase + rr;
+#pragma unroll
             for (int q = 0; q < RQ; ++q) {
                 const int k = lane + 64 * q;
-                y[q] = (k < R) ? M[i + In * k] : 0.0;
-            }
-            if (rk == 0) {
-#pragma unroll
-                for (int q = 0; q < RQ; ++q) y[q] = 0.0;
-            } else {
-                // Q^T y: H_0 first (xORMQR, side L, trans T)
-                for (int tq = 0; tq < R; ++tq) {
-                    const double tau = qtau[tq];
-                    if (tau == 0.0) continue;
-                    double part = 0.0;
-#pragma unroll
-                    for (int q = 0; q < RQ; ++q) {
-                        const int k = lane + 64 * q;
-                        if (k == tq) part += y[q];
-                        else if (k > tq && k < R) part += W[k + (int64_t)R * tq] * y[q];
-                    }
-                    const double f = tau * wave_sum(part);
-#pragma unroll
-                    for (int q = 0; q < RQ; ++q) {
-                        const int k = lane + 64 * q;
-                        if (k == tq) y[q] -= f;
-                        else if (k > tq && k < R) y[q] -= f * W[k + (int64_t)R * tq];
-                    }
-                }
-                // T11 y1 = (Q^T y)(0:rk), backward; y(rk:R) = 0
-#pragma unroll
-                for (int q = 0; q < RQ; ++q)
-                    if (lane + 64 * q >= rk) y[q] = 0.0;
-                for (int j = rk - 1; j >= 0; --j) {
-                    const double wj = bcast<RQ>(y, j) / W[j + (int64_t)R * j];
-#pragma unroll
-                    for (int q = 0; q < RQ; ++q) {
-                        const int k = lane + 64 * q;
-                        if (k == j) y[q] = wj;
-                        else if (k < j) y[q] -= W[k + (int64_t)R * j] * wj;
-                    }
-                }
-                // Z^T y: Z_0 first (xORMRZ, side L, trans T)
-                if (rk < R) {
-                    for (int tq = 0; tq < rk; ++tq) {
-                        const double tau = ztau[tq];
-                        if (tau == 0.0) continue;
-                        double part = 0.0;
-#pragma unroll
-                        for (int q = 0; q < RQ; ++q) {
-                            const int k = lane + 64 * q;
-                            if (k == tq) part += y[q];
-                            else if (k >= rk && k < R) part += W[tq + (int64_t)R * k] * y[q];
-                        }
-                        const double f = tau * wave_sum(part);
-#pragma unroll
-                        for (int q = 0; q < RQ; ++q) {
-                            const int k = lane + 64 * q;
-                            if (k == tq) y[q] -= f;
-                            else if (k >= rk && k < R) y[q] -= f * W[tq + (int64_t)R * k];
-                        }
-                    }
+                if (k < R) {
+                    X[i + In * pk[q]] = y[rr][q];
+                    Xs[(rbase + rr) * R + pk[q]] = y[rr][q];
                 }
             }
         }
-#pragma unroll
-        for (int q = 0; q < RQ; ++q) {
-            const int k = lane + 64 * q;
-            if (k < R) {
-                X[i + In * pk[q]] = y[q];
-                ss[q] += y[q] * y[q];
+        if (grams) {
+            __syncthreads();
+            for (int x = t; x < R * R; x += 256) {
+                const int r = x % R, r2 = x / R;
+                double sacc = 0.0;
+                for (int ii = 0; ii < nr; ++ii) sacc += Xs[ii * R + r] * Xs[ii * R + r2];
+                gp[x] = first_tile ? sacc : gp[x] + sacc;
             }
+        }
+        __syncthreads();
+        first_tile = false;
+    }
+}
+
+// row_norm + post_solve across blocks, after the row solves:
+// lambda_r = sqrt(sum_b gram_part[b](r,r)) (every block, fixed order); block k
+// writes its slice of G = S ./ (lambda lambda^T) (S = sum_b gram_part[b]) and of the
+// rows A = X ./ lambda, At; with `inner_part` it also sums <M, A .* lambda> over
+// its rows.
+__global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ gram_part, int nb,
+                                                  const double* __restrict__ X,
+                                                  const double* __restrict__ Msum, int64_t In, int R,
+                                                  int Rp, double* __restrict__ A,
+                                                  double* __restrict__ At, double* __restrict__ lam,
+                                                  double* __restrict__ Gout,
+                                                  double* __restrict__ inner_part,
+                                                  const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    extern __shared__ double lamS[];
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    const int64_t RR = (int64_t)R * R;
+    for (int r = t; r < R; r += 256) {
+        lamS[r] = sqrt(sum_parts_at(gram_part, RR, nb, (int64_t)r * R + r));
+        if (blockIdx.x == 0) lam[r] = lamS[r];
+    }
+    __syncthreads();
+    const int64_t gchunk = (RR + gridDim.x - 1) / gridDim.x;
+    const int64_t g0 = (int64_t)blockIdx.x * gchunk;
+    const int64_t g1 = g0 + gchunk < RR ? g0 + gchunk : RR;
+    for (int64_t x = g0 + t; x < g1; x += 256) {
+        const double sacc = sum_parts_at(gram_part, RR, nb, x);
+        const int r = (int)(x % R), r2 = (int)(x / R);
+        Gout[x] = sacc / (lamS[r] * lamS[r2]);
+    }
+    const int64_t rchunk = (In + gridDim.x - 1) / gridDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * rchunk;
+    const int64_t i1 = i0 + rchunk < In ? i0 + rchunk : In;
+    const int nr = (int)(i1 > i0 ? i1 - i0 : 0);
+    double inner = 0.0;
+    for (int x0 = t; x0 < nr * R; x0 += 256 * 4) {
+        double xv[4], mv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int x = x0 + 256 * k;
+            const bool ok = x < nr * R;
+            const int64_t off = ok ? i0 + (x % nr) + In * (x / nr) : 0;
+            xv[k] = ok ? X[off] : 0.0;
+            mv[k] = (ok && inner_part) ? Msum[off] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int x = x0 + 256 * k;
+            if (x >= nr * R) continue;
+            const int ii = x % nr, r = x / nr;
+            const int64_t i = i0 + ii;
+            const double a = xv[k] / lamS[r];
+            A[i + In * r] = a;
+            At[i * Rp + r] = a;
+            if (inner_part) inner += mv[k] * (a * lamS[r]);
         }
     }
-#pragma unroll
-    for (int q = 0; q < RQ; ++q) red[wave][lane + 64 * q] = ss[q];
-    __syncthreads();
-    // block partial indexed by the un-permuted column
-    for (int x = threadIdx.x; x < 64 * RQ; x += 256) {
-        if (x < R) {
-            const double v = red[0][x] + red[1][x] + red[2][x] + red[3][x];
-            const int col = fallback ? jpvt[x] : piv[x];
-            sumsq_part[(int64_t)blockIdx.x * R + col] = v;
+    if (inner_part) {
+        inner = block_sum<256>(inner, red);
+        if (t == 0) inner_part[blockIdx.x] = inner;
+    }
+}
+
+// After the last mode of a sweep: the fit from <T,X> (sum of the finalize blocks'
+// partials, fixed order) and ||X||^2 = lambda^T (G_0 .* ... .* G_{N-1}) lambda, then
+// the check update (FitCheck) or the sweep count (NoCheck).
+__global__ __launch_bounds__(256) void k_fit(const double* __restrict__ inner_part, int nbf, int R,
+                                             GramSet gs, const double* __restrict__ lam,
+                                             double* __restrict__ scal, DevCheck* __restrict__ chk,
+                                             double* __restrict__ trace, int trace_cap) {
+    if (chk->done) return;
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    if (chk->kind == ITCPD_CHECK_NONE) {
+        if (t == 0) {
+            chk->iter += 1;
+            chk->sweeps += 1;
+            if (chk->iter == chk->max_counter) {
+                chk->iter = 0;
+                chk->done = 1;
+            }
         }
+        return;
+    }
+    double fs = 0.0;
+    for (int x0 = t; x0 < R * R; x0 += 256 * 16) {
+        double h[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) h[k] = 1.0;
+        for (int m = 0; m < gs.N; ++m) {
+            const double* __restrict__ G = gs.G[m];
+            double g[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int x = x0 + 256 * k;
+                g[k] = G[x < R * R ? x : 0];
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) h[k] *= g[k];
+        }
+        double l1[16], l2[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int x = x0 + 256 * k;
+            l1[k] = lam[(x < R * R ? x : 0) % R];
+            l2[k] = lam[(x < R * R ? x : 0) / R];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (x0 + 256 * k < R * R) fs += h[k] * (l1[k] * l2[k]);
+    }
+    const double fact_square = block_sum<256>(fs, red);
+    if (t == 0) {
+        const double inner = sum_parts_at(inner_part, 1, nbf, 0);
+        check_update(chk, inner, fact_square, scal, trace, trace_cap);
     }
 }
 
@@ -726,45 +1149,7 @@ __global__ __launch_bounds__(256) void k_gram_final(const double* __restrict__ g
     if (t == 0) {
         double inner = 0.0;
         for (int b = 0; b < nb; ++b) inner += inner_part[b];
-        const double ref = chk->ref_norm;
-        const double normResidual = sqrt(fabs(ref * ref + fact_square - 2.0 * fabs(inner)));
-        const double fit = 1.0 - normResidual / ref;
-        scal[0] = inner;
-        scal[1] = fact_square;
-        scal[2] = fit;
-        if (chk->kind == ITCPD_CHECK_FIT) {
-            chk->iter += 1;
-            const double dfit = fabs(chk->lastfit - fit);
-            chk->lastfit = fit;
-            if (trace && chk->sweeps < trace_cap) trace[chk->sweeps] = fit;
-            chk->sweeps += 1;
-            if (fit != fit) {
-                chk->nan_flag = 1;
-                chk->done = 1;
-                return;
-            }
-            if (dfit < chk->tolerance) {
-                chk->counter += 1;
-                if (chk->counter >= 2) {
-                    chk->total_iter = chk->iter;
-                    chk->iter = 0;
-                    chk->counter = 0;
-                    chk->final_fit = chk->lastfit;
-                    chk->lastfit = 0.0;
-                    chk->done = 1;
-                    return;
-                }
-            } else {
-                chk->counter = 0;
-            }
-            if (chk->iter >= chk->max_counter) {
-                chk->total_iter = chk->iter;
-                chk->iter = 0;
-                chk->counter = 0;
-                chk->final_fit = chk->lastfit;
-                chk->lastfit = 0.0;
-            }
-        }
+        check_update(chk, inner, fact_square, scal, trace, trace_cap);
     }
 }
 
@@ -878,44 +1263,129 @@ int solve_blocks(int64_t In) {
     return (int)(nb < 128 ? nb : 128);
 }
 
-template <int RQ>
-int run_solve_rows(itcpd_plan* p, int64_t In, const double* Msrc, int nb, bool u_lds) {
-    const size_t lds = u_lds ? (size_t)p->R * (p->R + 1) * 8 : 0;
-    hipLaunchKernelGGL(k_solve_rows<RQ>, dim3(nb), dim3(256), lds, p->stream, Msrc, In, (int)p->R,
-                       p->U, p->piv, p->finfo, p->qr_aux, p->qr_aux + p->R * p->R,
-                       p->qr_aux + p->R * p->R + p->R, (const int*)(p->piv + p->R), p->X,
-                       p->sumsq_part, u_lds ? 1 : 0, p->chk);
-    ITCPD_HIP(hipGetLastError());
-    return ITCPD_OK;
-}
-
 }  // namespace
 
-int launch_factor(itcpd_plan* p, int mode) {
+int launch_factor(itcpd_plan* p, int mode) { return launch_factor_on(p, mode, p->stream); }
+
+int launch_factor_on(itcpd_plan* p, int mode, hipStream_t st) {
     const int R = (int)p->R;
-    const bool lds = (size_t)R * R * 8 <= 128 * 1024;
     double* W = p->qr_aux;
     double* qtau = W + (size_t)R * R;
     double* ztau = qtau + R;
     double* vwork = ztau + R;
-    hipLaunchKernelGGL(k_factor, dim3(1), dim3(FACTOR_THREADS), lds ? (size_t)R * R * 8 : 0,
-                       p->stream, grams_of(p), mode, R, 1e-6, p->U, p->piv, p->finfo, W, qtau, ztau,
-                       p->piv + R, vwork, lds ? 1 : 0, p->chk);
+    if (R <= 64) {
+        hipLaunchKernelGGL(k_factor64, dim3(1), dim3(256), (size_t)(((R * R + 1) & ~1) + 64 * 66) * 8,
+                           st, grams_of(p), mode, R, 1e-6, p->U, p->invdiag, p->piv, p->finfo, W,
+                           qtau, ztau, p->piv + R, vwork, p->chk);
+    } else {
+        // right-looking, all threads; V in LDS up to R = 128, else in the U buffer
+        const bool lds = R <= 128;
+        hipLaunchKernelGGL(k_factor, dim3(1), dim3(FACTOR_THREADS), lds ? (size_t)R * R * 8 : 0, st,
+                           grams_of(p), mode, R, 1e-6, p->U, p->invdiag, p->piv, p->finfo, W, qtau,
+                           ztau, p->piv + R, vwork, lds ? 1 : 0, p->chk);
+    }
     ITCPD_HIP(hipGetLastError());
     return ITCPD_OK;
 }
 
-int launch_solve_rows(itcpd_plan* p, int mode, const double* Msrc) {
+namespace {
+
+// rows per wave in the row solves (fewer for large R to bound registers)
+template <int RQ>
+constexpr int rpw_of() { return RQ <= 4 ? 4 : (RQ == 8 ? 2 : 1); }
+int rpb_of(int R) { return 4 * (R <= 256 ? 4 : (R <= 512 ? 2 : 1)); }
+
+// Block count of the row solves, capped so that the per-block Gram partials stay
+// within 2^25 doubles.
+int solve_nb(int64_t In, int R) {
+    int64_t cap = ((int64_t)1 << 25) / ((int64_t)R * R);
+    cap = cap < 1 ? 1 : (cap > 256 ? 256 : cap);
+    const int64_t nb = cdiv(In, rpb_of(R));
+    return (int)(nb < cap ? nb : cap);
+}
+int finalize_nb(int64_t In) {
+    const int64_t nb = cdiv(In, 32);
+    return (int)(nb < 16 ? (nb < 1 ? 1 : nb) : 16);
+}
+
+template <int RQ>
+int run_solve_k(itcpd_plan* p, int mode, const double* Msrc, int nparts, int64_t stride,
+                bool grams, bool keep_m) {
     const int64_t In = p->dims[mode];
-    const int nb = solve_blocks(In);
-    const bool u_lds = (size_t)p->R * (p->R + 1) * 8 <= 130 * 1024;
     const int R = (int)p->R;
-    if (R <= 64) return run_solve_rows<1>(p, In, Msrc, nb, u_lds);
-    if (R <= 128) return run_solve_rows<2>(p, In, Msrc, nb, u_lds);
-    if (R <= 256) return run_solve_rows<4>(p, In, Msrc, nb, false);
-    if (R <= 512) return run_solve_rows<8>(p, In, Msrc, nb, false);
-    if (R <= 1024) return run_solve_rows<16>(p, In, Msrc, nb, false);
-    return run_solve_rows<32>(p, In, Msrc, nb, false);
+    const int nb = solve_nb(In, R);
+    const bool u_lds = R <= 128;
+    constexpr int RPW = rpw_of<RQ>();
+    const size_t lds = (size_t)4 * RPW * R * 8 + (u_lds ? (size_t)R * (R + 1) * 8 : 0);
+    double* W = p->qr_aux;
+    hipLaunchKernelGGL((k_solve<RQ, RPW>), dim3(nb), dim3(256), lds, p->stream, Msrc, In, R, p->U, p->invdiag, p->piv, p->finfo, W, W + (size_t)R * R,
+                       W + (size_t)R * R + R, (const int*)(p->piv + R), u_lds ? 1 : 0, p->X,
+                       keep_m ? p->Msum : (double*)nullptr, p->gram_part, grams ? 1 : 0, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+int run_apply(itcpd_plan* p, int mode, const double* Msrc, int nparts, int64_t stride, bool grams,
+              bool keep_m) {
+    const int R = (int)p->R;
+    if (R <= 64) return run_solve_k<1>(p, mode, Msrc, nparts, stride, grams, keep_m);
+    if (R <= 128) return run_solve_k<2>(p, mode, Msrc, nparts, stride, grams, keep_m);
+    if (R <= 256) return run_solve_k<4>(p, mode, Msrc, nparts, stride, grams, keep_m);
+    if (R <= 512) return run_solve_k<8>(p, mode, Msrc, nparts, stride, grams, keep_m);
+    return run_solve_k<16>(p, mode, Msrc, nparts, stride, grams, keep_m);
+}
+
+}  // namespace
+
+int launch_solve(itcpd_plan* p, int mode, const double* Msrc, int nparts, int64_t stride,
+                 bool finalize, bool fit_after) {
+    const int R = (int)p->R;
+    int rc;
+    if (nparts > 1) {
+        const int64_t n = p->dims[mode] * p->R;
+        if ((rc = launch_sum_parts(p, MSrc{Msrc, nparts, stride}, n, p->M))) return rc;
+        Msrc = p->M;
+        nparts = 1;
+    }
+    rc = run_apply(p, mode, Msrc, nparts, stride, finalize, fit_after);
+    if (rc || !finalize) return rc;
+    const int64_t In = p->dims[mode];
+    const int nb = solve_nb(In, R);
+    const int nbf = finalize_nb(In);
+    hipLaunchKernelGGL(k_finalize, dim3(nbf), dim3(256), (size_t)R * 8, p->stream, p->gram_part, nb,
+                       p->X, p->Msum, In, R, (int)p->Rp, p->A[mode], p->At[mode], p->lam,
+                       p->G[mode], fit_after ? p->inner_part : (double*)nullptr, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    if (!fit_after) return ITCPD_OK;
+    hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, p->stream, p->inner_part, nbf, R, grams_of(p),
+                       p->lam, p->scal, p->chk, p->fit_trace, p->trace_cap);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+int launch_update_mode(itcpd_plan* p, int mode, bool fit_after, const double* Msrc, int nparts,
+                       int64_t stride) {
+    int rc;
+    if ((rc = launch_factor(p, mode))) return rc;
+    return launch_solve(p, mode, Msrc, nparts, stride, true, fit_after);
+}
+
+// V_mode depends only on the Grams of the other modes, which are final once the
+// previous mode's update has run on the main stream: the factorisation is forked
+// onto the side stream there and overlaps the MTTKRP of `mode`.
+int fork_factor(itcpd_plan* p, int mode) {
+    if (!p->side_stream) return launch_factor(p, mode);
+    ITCPD_HIP(hipEventRecord(p->ev_main[mode], p->stream));
+    ITCPD_HIP(hipStreamWaitEvent(p->side_stream, p->ev_main[mode], 0));
+    int rc;
+    if ((rc = launch_factor_on(p, mode, p->side_stream))) return rc;
+    ITCPD_HIP(hipEventRecord(p->ev_side[mode], p->side_stream));
+    return ITCPD_OK;
+}
+int join_factor(itcpd_plan* p, int mode) {
+    if (!p->side_stream) return ITCPD_OK;
+    ITCPD_HIP(hipStreamWaitEvent(p->stream, p->ev_side[mode], 0));
+    return ITCPD_OK;
 }
 
 int launch_normalize(itcpd_plan* p, int mode, const double* Xsrc, bool fit_after) {
@@ -934,13 +1404,6 @@ int launch_normalize(itcpd_plan* p, int mode, const double* Xsrc, bool fit_after
                        p->scal, p->chk, p->fit_trace, p->trace_cap);
     ITCPD_HIP(hipGetLastError());
     return ITCPD_OK;
-}
-
-int launch_update_mode(itcpd_plan* p, int mode, bool fit_after) {
-    int rc;
-    if ((rc = launch_factor(p, mode))) return rc;
-    if ((rc = launch_solve_rows(p, mode, p->M))) return rc;
-    return launch_normalize(p, mode, p->X, fit_after);
 }
 
 // Gram cache seeding (optimizers/als_optimizers/standard/tensor.jl:12) and the

@@ -148,7 +148,15 @@ def mttkrp(T: np.ndarray, factors, n: int, form: str = "tree") -> np.ndarray:
     form="krp": explicit Khatri-Rao product, then one contraction ``target * krp``
         (algorithms/.../standard/tensor.jl:12-20).
     form="tree": the same sum computed by BLAS-backed pairwise contractions
-        (identical mathematics, different rounding); used for large sizes.
+        (identical mathematics, different rounding).
+    form="gemm": the same sum on the column-major layout: T viewed as
+        (before x I_n x after), one GEMM against the Khatri-Rao product of the
+        trailing modes, then the leading modes (used for large sizes).
+    form="slices": the krpfree loop over rank slices (had_contract.jl:111-121)
+        executed the way a column-major contraction engine runs each slice: the
+        trailing modes are contracted as GEMVs on the last axis, then the leading
+        ones on the first axis, so T is read once per slice with no permutation
+        copy.  This is the form the CPU throughput baseline times.
     """
     N = T.ndim
     R = factors[0].shape[1]
@@ -162,6 +170,18 @@ def mttkrp(T: np.ndarray, factors, n: int, form: str = "tree") -> np.ndarray:
                 x = _contract_vec(x, factors[m][:, r], m)
             out[:, r] = x
         return out
+    if form == "slices":
+        Tf = np.asfortranarray(T)
+        dims = T.shape
+        out = np.empty((dims[n], R))
+        for r in range(R):
+            x = Tf
+            for m in range(N - 1, n, -1):                 # last axis first
+                x = x.reshape(-1, dims[m], order="F") @ factors[m][:, r]
+            for m in range(n):                            # then the leading axes
+                x = factors[m][:, r] @ x.reshape(dims[m], -1, order="F")
+            out[:, r] = x
+        return out
     if form == "krp":
         # KRP(i_others..., r) = prod_m A_m(i_m, r)   (had_contract over the factors)
         K = np.ones((1, R))
@@ -172,6 +192,22 @@ def mttkrp(T: np.ndarray, factors, n: int, form: str = "tree") -> np.ndarray:
         # C-order reshape: rows enumerate (i_others[0], ..., i_others[-1]), last fastest
         K = K.reshape(*shape, R)
         return np.tensordot(T, K, axes=(others, list(range(len(others)))))
+    if form == "gemm":
+        before = list(range(n))
+        after = list(range(n + 1, N))
+        A_ = int(np.prod([T.shape[m] for m in before])) if before else 1
+        B_ = int(np.prod([T.shape[m] for m in after])) if after else 1
+        In = T.shape[n]
+        Tf = np.asfortranarray(T)
+        Ka = _krp_fortran([factors[m] for m in after], R)      # B_ x R, first mode fastest
+        Kb = _krp_fortran([factors[m] for m in before], R)     # A_ x R
+        if B_ >= A_:
+            # GEMM over the trailing modes, then the leading ones
+            Q = Tf.reshape(A_ * In, B_, order="F") @ Ka         # (A_ * In) x R
+            return np.einsum("air,ar->ir", Q.reshape(A_, In, R, order="F"), Kb)
+        # GEMM over the leading modes, then the trailing ones
+        Q = Tf.reshape(A_, In * B_, order="F").T @ Kb           # (In * B_) x R
+        return np.einsum("ibr,br->ir", Q.reshape(In, B_, R, order="F"), Ka)
     if form == "tree":
         if N == 1:
             return T[:, None] * np.ones((1, R))
@@ -187,6 +223,15 @@ def mttkrp(T: np.ndarray, factors, n: int, form: str = "tree") -> np.ndarray:
             rest.pop(ax)
         return P
     raise ValueError(form)
+
+
+def _krp_fortran(fs, R):
+    """Khatri-Rao product with the first factor's index fastest (rows = column-major
+    multi-index); an empty list gives a 1 x R row of ones."""
+    K = np.ones((1, R))
+    for f in fs:
+        K = (f[:, None, :] * K[None, :, :]).reshape(-1, R)
+    return K
 
 
 def _sub(nd: int, ax: int) -> str:
@@ -432,7 +477,7 @@ def optimize(T, cp: CPD, alg=None, check=None, verbose=False, trace: Optional[di
     alg = KRPFreeNormal() if alg is None else alg
     check = NoCheck(100) if check is None else check
     if form is None:
-        form = "tree"
+        form = "gemm"
     st = ALSState(grams(cp.factors))
     it = check.iter
     lam = cp.lam.copy()

@@ -1,0 +1,332 @@
+"""GPU parity tests: the HIP engine (through the C ABI) against the CPU oracle and
+the committed golden fixtures.  Tolerances are relative Frobenius errors; the
+plain FP64 path is held to 1e-10 (factor and lambda), fits to 1e-10 absolute."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import rng as ORNG
+from conftest import lowrank
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+TOL = 1e-10
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def factors_err(got, ref):
+    return max(relerr(a, b) for a, b in zip(got.factors, ref.factors))
+
+
+def run_gpu(itcpd, T, init, check):
+    with itcpd.ALSPlan(T, init.rank) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        info = plan.run(check)
+        return plan.get_cpd(), info
+
+
+def test_smoke():
+    import __graft_entry__ as G
+    G.smoke()
+
+
+# --------------------------------------------------------------- golden files
+@pytest.mark.parametrize("name", sorted(p.stem for p in GOLD.glob("*.npz")))
+def test_golden_fixture(itcpd, name):
+    z = np.load(GOLD / f"{name}.npz")
+    N = z["T"].ndim
+    init = O.CPD([z[f"init_{n}"] for n in range(N)], z["init_lam"])
+    if int(z["check_kind"]) == 1:
+        check = itcpd.FitCheck(float(z["tol"]), int(z["max_counter"]), float(z["ref_norm"]))
+    else:
+        check = itcpd.NoCheck(int(z["max_counter"]))
+    got, info = run_gpu(itcpd, z["T"], init, check)
+    for n in range(N):
+        assert relerr(got.factors[n], z[f"out_{n}"]) < TOL
+    assert relerr(got.lam, z["out_lam"]) < TOL
+    if int(z["check_kind"]) == 1:
+        np.testing.assert_allclose(info["fits"], z["fits"], rtol=0, atol=TOL)
+        assert check.total_iter == int(z["total_iter"])
+        assert abs(check.final_fit - float(z["final_fit"])) < TOL
+
+
+# ------------------------------------------------------ sweeps vs the oracle
+SHAPES = [((12, 10, 8), 3), ((64, 64, 64), 10), ((33, 17, 29), 7), ((40, 30, 20), 16),
+          ((64, 48, 40), 32), ((96, 80, 72), 64), ((50, 60, 70), 80), ((30, 20), 4),
+          ((20, 18, 16, 14), 6), ((16, 12, 10, 8), 20), ((9, 8, 7, 6, 5), 4),
+          ((128, 128, 64), 24), ((100, 90, 110), 40), ((40, 30, 25), 150), ((20, 30, 40), 128)]
+
+
+@pytest.mark.parametrize("dims,R", SHAPES, ids=[f"{'x'.join(map(str, d))}-R{r}" for d, r in SHAPES])
+def test_sweeps_match_oracle(itcpd, dims, R):
+    T, _ = lowrank(dims, min(R, 8), 1e-3, 1)
+    init = O.random_CPD(T, R, np.random.default_rng(2))
+    got, _ = run_gpu(itcpd, T, init, itcpd.NoCheck(6))
+    ref = O.als_optimize(T, init, check=O.NoCheck(6))
+    assert factors_err(got, ref) < TOL
+    assert relerr(got.lam, ref.lam) < TOL
+
+
+def test_float32_tensor(itcpd):
+    T, _ = lowrank((40, 36, 28), 5, 1e-3, 3)
+    T32 = T.astype(np.float32)
+    init = O.random_CPD(T, 12, np.random.default_rng(4))
+    got, _ = run_gpu(itcpd, T32, init, itcpd.NoCheck(5))
+    ref = O.als_optimize(T32.astype(np.float64), init, check=O.NoCheck(5))
+    assert factors_err(got, ref) < TOL
+
+
+def test_fitcheck_trace_and_state(itcpd):
+    # BASELINE config 1 restated: 64^3, rank 10 + N(0, 1e-3), FitCheck(1e-6, 100, norm(T))
+    T, _ = lowrank((64, 64, 64), 10, 1e-3, 5)
+    init = O.random_CPD(T, 10, np.random.default_rng(6))
+    nrm = float(np.linalg.norm(T))
+    cg = itcpd.FitCheck(1e-6, 100, nrm)
+    got, info = run_gpu(itcpd, T, init, cg)
+    cc = O.FitCheck(1e-6, 100, nrm)
+    tr = {}
+    ref = O.als_optimize(T, init, check=cc, trace=tr)
+    assert cg.total_iter == cc.total_iter and info["converged"]
+    assert len(info["fits"]) == len(tr["fits"])
+    np.testing.assert_allclose(info["fits"], tr["fits"], rtol=0, atol=TOL)
+    assert abs(cg.final_fit - cc.final_fit) < TOL
+    assert (cg.iter, cg.counter, cg.lastfit) == (cc.iter, cc.counter, cc.lastfit)
+    assert factors_err(got, ref) < TOL
+
+
+def test_fitcheck_max_iterations_bookkeeping(itcpd):
+    T, _ = lowrank((20, 18, 16), 4, 1e-1, 7)
+    init = O.random_CPD(T, 4, np.random.default_rng(8))
+    nrm = float(np.linalg.norm(T))
+    cg, cc = itcpd.FitCheck(1e-14, 5, nrm), O.FitCheck(1e-14, 5, nrm)
+    _, info = run_gpu(itcpd, T, init, cg)
+    O.als_optimize(T, init, check=cc)
+    assert info["sweeps"] == 5 and not info["converged"]
+    assert (cg.total_iter, cg.iter, cg.counter, cg.lastfit) == (5, 0, 0, 0.0)
+    assert abs(cg.final_fit - cc.final_fit) < TOL
+
+
+def test_check_object_reused_across_calls(itcpd):
+    # decompose.jl:53 reuses a check after resetting iter; the state carries over
+    T, _ = lowrank((24, 20, 16), 4, 1e-3, 9)
+    init = O.random_CPD(T, 4, np.random.default_rng(10))
+    nrm = float(np.linalg.norm(T))
+    cg, cc = itcpd.FitCheck(1e-8, 30, nrm), O.FitCheck(1e-8, 30, nrm)
+    with itcpd.ALSPlan(T, 4) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        plan.run(cg)
+        mid = plan.get_cpd()
+        plan.run(cg)
+        got = plan.get_cpd()
+    mid_c = O.als_optimize(T, init, check=cc)
+    got_c = O.als_optimize(T, mid_c, check=cc)
+    assert factors_err(mid, mid_c) < TOL and factors_err(got, got_c) < TOL
+    assert (cg.total_iter, cg.final_fit) == pytest.approx((cc.total_iter, cc.final_fit), abs=TOL)
+
+
+# ------------------------------------------------------------------ hooks
+def test_hooks_match_oracle(itcpd):
+    T, _ = lowrank((30, 25, 20, 15), 5, 1e-2, 11)
+    init = O.random_CPD(T, 6, np.random.default_rng(12))
+    G = O.grams(init.factors)
+    with itcpd.ALSPlan(T, 6) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        for n in range(4):
+            assert np.abs(plan.compute_krp(n) - O.compute_krp(G, n)).max() < 1e-14
+            assert relerr(plan.mttkrp(n), O.mttkrp(T, init.factors, n)) < 1e-13
+        M = O.mttkrp(T, init.factors, 0)
+        X, fb = plan.solve_ls(0, M)
+        Xc, fbc = O.solve_ls_problem(O.compute_krp(G, 0), M)
+        assert fb == fbc and relerr(X, Xc) < 1e-12
+        plan.row_norm_post_solve(0, Xc)
+        cp = plan.get_cpd()
+        A0, lam = O.row_norm(Xc)
+        assert relerr(cp.factors[0], A0) < 1e-14 and relerr(cp.lam, lam) < 1e-14
+        assert np.abs(plan.compute_krp(1) - O.compute_krp([A0.T @ A0] + G[1:], 1)).max() < 1e-13
+        # fit hook (fit_check.jl:24-32) with the current CPD
+        factors = [A0] + init.factors[1:]
+        Mlast = O.mttkrp(T, factors, 3)
+        inner, fs, fit = plan.fit(Mlast, float(np.linalg.norm(T)))
+        want_inner = float(np.sum(Mlast * (factors[3] * lam[None, :])))
+        want_fs = O.norm_factors(O.grams(factors), lam)
+        assert abs(inner - want_inner) < 1e-10 * abs(want_inner)
+        assert abs(fs - want_fs) < 1e-10 * abs(want_fs)
+        assert abs(fit - O.fit_from_parts(np.linalg.norm(T), want_inner, want_fs)) < 1e-10
+
+
+@pytest.mark.parametrize("R,pos,small", [(64, 37, 5e-7), (64, 37, 2e-6), (100, 71, 5e-7),
+                                         (130, 3, 5e-7), (130, 3, 3e-6), (200, 150, 5e-7)])
+def test_pivot_threshold_and_fallback(itcpd, R, pos, small):
+    # V = diag(d^2): the pivoted Cholesky takes the pivots in descending order, so the
+    # QR fallback triggers exactly when the tiny pivot is <= 1e-6 (a wrong argmax
+    # would take it first, where only > 0 is checked)
+    rng = np.random.default_rng(R)
+    d = 1 + np.arange(R) / R
+    d[pos] = np.sqrt(small)
+    A0 = rng.standard_normal((10, R))
+    with itcpd.ALSPlan(rng.standard_normal((10, R)), R) as plan:
+        plan.set_cpd(itcpd.CPD([A0, np.diag(d)], np.ones(R)))
+        M = rng.standard_normal((10, R))
+        X, fb = plan.solve_ls(0, M)
+    Xc, fbc = O.solve_ls_problem(np.diag(d) @ np.diag(d), M)
+    assert fb == fbc == (small <= 1e-6)
+    assert relerr(X, Xc) < 1e-12
+
+
+@pytest.mark.parametrize("dims,R", [((12, 10, 8), 5), ((30, 20, 25), 12), ((20, 18, 16), 70)])
+def test_rank_deficient_sweeps_use_qr(itcpd, dims, R):
+    T = np.random.default_rng(3).standard_normal(dims)
+    init = O.random_CPD(T, R, np.random.default_rng(4))
+    for f in init.factors:
+        f[:, 1] = f[:, 0]
+    got, _ = run_gpu(itcpd, T, init, itcpd.NoCheck(4))
+    tr = {}
+    ref = O.optimize(T, init, check=O.NoCheck(4), trace=tr)
+    assert tr["fallbacks"] == 12
+    assert factors_err(got, ref) < 1e-9
+
+
+# --------------------------------------------------------- init, norm, NaN
+def test_random_cpd_matches_philox_restatement(itcpd):
+    dims, R, seed = (37, 20, 11), 9, 12345
+    cp = itcpd.random_CPD(dims, R, rng=seed)
+    f, lam = ORNG.random_factors_philox(dims, R, seed)
+    assert factors_err(cp, O.CPD(f, lam)) < 1e-13
+    assert relerr(cp.lam, lam) < 1e-13
+    for a in cp.factors:
+        np.testing.assert_allclose((a ** 2).sum(axis=0), 1.0, rtol=1e-13)
+    cp2 = itcpd.random_CPD(dims, R, rng=seed)
+    assert cp.isequal(cp2)
+    assert not cp.isequal(itcpd.random_CPD(dims, R, rng=seed + 1))
+
+
+def test_tensor_norm(itcpd):
+    T = np.random.default_rng(13).standard_normal((33, 21, 17))
+    assert abs(itcpd.tensor_norm(T) - np.linalg.norm(T)) < 1e-12 * np.linalg.norm(T)
+    assert abs(itcpd.tensor_norm(T.astype(np.float32)) - np.linalg.norm(T.astype(np.float32)
+                                                                        .astype(np.float64))) < 1e-9
+
+
+def test_nan_fit_raises(itcpd):
+    T, _ = lowrank((10, 9, 8), 3, 1e-2, 14)
+    T[3, 4, 5] = np.nan
+    init = O.random_CPD(T, 3, np.random.default_rng(15))
+    chk = itcpd.FitCheck(1e-6, 20, 1.0)
+    with pytest.raises(itcpd.api.ALSNaNError, match="Error NAN"):
+        run_gpu(itcpd, T, init, chk)
+    assert np.isnan(chk.lastfit) and chk.iter == 1
+
+
+def test_errors_are_status_codes(itcpd):
+    L = itcpd._lib
+    T = np.ones((4, 4, 4))
+    with itcpd.ALSPlan(T, 2) as plan:
+        with pytest.raises(L.ITCPDError) as e:
+            plan.run(itcpd.NoCheck(3))
+        assert e.value.code == L.ITCPD_E_STATE
+        plan.random_cpd(1)
+        with pytest.raises(L.ITCPDError) as e:
+            plan.mttkrp(7)
+        assert e.value.code == L.ITCPD_E_ARG
+    with pytest.raises(L.ITCPDError) as e:
+        itcpd.ALSPlan(T, 2, prec=L.ITCPD_BF16X3 + 5)
+    assert e.value.code in (L.ITCPD_E_UNSUPPORTED, L.ITCPD_E_ARG)
+
+
+# ------------------------------------------- the reference's own test bars
+@pytest.fixture(scope="module")
+def a203040():
+    return np.random.default_rng(100).standard_normal((20, 30, 40))
+
+
+def _rel(A):
+    A64 = A.astype(np.float64)
+    return lambda cp: np.linalg.norm(O.reconstruct(cp.factors, cp.lam) - A64) / np.linalg.norm(A64)
+
+
+def test_cp_als_float64_bars(itcpd, a203040):
+    """test/cp_als.jl:1-45 (the real-valued Float64 cases on the normal-equation path)."""
+    A = a203040
+    rel = _rel(A)
+    assert rel(itcpd.decompose(A, 400)) < 1e-7
+    with pytest.raises(TypeError):
+        itcpd.decompose(A, 400, solver=A)
+    with pytest.raises(RuntimeError, match="OptimizerError"):
+        itcpd.decompose(A, 4, solver=itcpd.CPDOptimizer())
+    check = itcpd.FitCheck(1e-6, 100, float(np.linalg.norm(A)))
+    assert rel(itcpd.decompose(A, 400, check=check)) < 1e-5
+    cp_A = itcpd.random_CPD(A, 400)
+    assert rel(itcpd.als_optimize(A, cp_A, check=check)) < 1e-5
+    cp_A = itcpd.random_CPD(A, 400)
+    assert rel(itcpd.als_optimize(A, cp_A, alg=itcpd.KRPNormal())) < 5e-7
+    assert rel(itcpd.als_optimize(A, cp_A, alg=itcpd.KRPNormal(), check=itcpd.NoCheck(10))) < 1e-1
+    assert rel(itcpd.als_optimize(A, cp_A, alg=itcpd.KRPNormal(), check=check)) < 1e-5
+    assert rel(itcpd.als_optimize(A, cp_A, alg=itcpd.KRPFreeNormal())) < 5e-7
+    assert rel(itcpd.als_optimize(A, cp_A, alg=itcpd.KRPFreeNormal(), check=check)) < 1e-5
+
+
+def test_cp_als_float32_bars(itcpd):
+    """test/cp_als.jl:52-96: an exact rank-4 Float32 target decomposed at rank 400."""
+    f, lam = O.random_factors((20, 30, 40), 4, np.random.default_rng(101))
+    A = O.reconstruct(f, lam).astype(np.float32)
+    rel = _rel(A)
+    nA = float(np.linalg.norm(A.astype(np.float64)))
+    check = itcpd.FitCheck(1e-6, 100, nA)
+    assert rel(itcpd.decompose(A, 400, check=check)) < 1e-3
+    check = itcpd.FitCheck(1e-6, 100, nA)
+    cp_A = itcpd.random_CPD(A, 400)
+    assert rel(itcpd.als_optimize(A, cp_A, check=check)) < 5e-5
+    assert rel(itcpd.als_optimize(A, cp_A, alg=itcpd.KRPNormal())) < 5e-5
+    assert rel(itcpd.als_optimize(A, cp_A, alg=itcpd.KRPNormal(), check=itcpd.NoCheck(10))) < 1e-1
+    assert rel(itcpd.als_optimize(A, cp_A, alg=itcpd.KRPNormal(), check=check)) < 5e-5
+    assert rel(itcpd.als_optimize(A, cp_A, alg=itcpd.KRPFreeNormal())) < 5e-5
+    assert rel(itcpd.als_optimize(A, cp_A, alg=itcpd.KRPFreeNormal(), check=check)) < 5e-5
+
+
+def test_rank_adaptive_decompose(itcpd, a203040):
+    # cp_als.jl:106-115: decompose(A, 1e-3, 400; FitCheck(1e-4, 100), start 200, step 200)
+    A = a203040
+    out = itcpd.decompose(A, 1e-3, 400, check=itcpd.FitCheck(1e-4, 100, float(np.linalg.norm(A))),
+                          start_rank=200, rank_step=200)
+    assert np.linalg.norm(O.reconstruct(out.factors, out.lam) - A) / np.linalg.norm(A) < 1e-3
+
+
+def test_input_cpd_not_mutated_and_verbose(itcpd, capsys):
+    T, _ = lowrank((12, 11, 10), 3, 1e-2, 16)
+    cp = itcpd.random_CPD(T, 3, rng=5)
+    before = cp.copy()
+    chk = itcpd.FitCheck(1e-12, 3, float(np.linalg.norm(T)))
+    itcpd.als_optimize(T, cp, check=chk, verbose=True)
+    assert cp.isequal(before)
+    lines = capsys.readouterr().out.strip().splitlines()
+    assert len(lines) == 3 and lines[0].startswith("3\t 1 \t ")
+
+
+# ------------------------------------------------ full-size plain FP64 parity
+def test_plain_fp64_512_cubed_50_sweeps(itcpd):
+    """512^3 FP64, rank-64 N(0,1) factor sum + N(0, 1e-2) noise,
+    rank 64, 50 sweeps from the same init: factors and fit within 1e-10."""
+    I, R = 512, 64
+    rng = np.random.default_rng(2024)
+    A = [rng.standard_normal((I, R)) for _ in range(3)]
+    K = (A[2][:, None, :] * A[1][None, :, :]).reshape(I * I, R)
+    Tc = (K @ A[0].T).reshape(I, I, I)
+    del K
+    Tc += 1e-2 * rng.standard_normal(Tc.shape)
+    T = Tc.transpose(2, 1, 0)
+    init = O.random_CPD(T, R, np.random.default_rng(7))
+    nrm = float(np.linalg.norm(Tc.ravel()))
+    cg = itcpd.FitCheck(0.0, 50, nrm)
+    got, info = run_gpu(itcpd, T, init, cg)
+    cc = O.FitCheck(0.0, 50, nrm)
+    tr = {}
+    ref = O.als_optimize(T, init, check=cc, trace=tr)
+    assert info["sweeps"] == 50
+    assert factors_err(got, ref) < TOL
+    assert relerr(got.lam, ref.lam) < TOL
+    np.testing.assert_allclose(info["fits"], tr["fits"], rtol=0, atol=TOL)

@@ -38,7 +38,7 @@ def test_row_norm_unit_columns(shape):
 
 # ------------------------------------------------------------------ mttkrp
 @pytest.mark.parametrize("dims", [(4, 3, 5), (3, 4, 2, 3), (5, 6)])
-@pytest.mark.parametrize("form", ["krpfree", "krp", "tree"])
+@pytest.mark.parametrize("form", ["krpfree", "krp", "tree", "gemm", "slices"])
 def test_mttkrp_brute_force(dims, form):
     rng = np.random.default_rng(len(dims))
     T = rng.standard_normal(dims)
@@ -55,7 +55,7 @@ def test_mttkrp_rank1_closed_form():
     T = np.einsum("i,j,k->ijk", a, b, c)
     F = [rng.standard_normal((d, 3)) for d in (6, 5, 4)]
     want = np.outer(a, (b @ F[1]) * (c @ F[2]))
-    for form in ("krpfree", "krp", "tree"):
+    for form in ("krpfree", "krp", "tree", "gemm", "slices"):
         np.testing.assert_allclose(O.mttkrp(T, F, 0, form), want, rtol=1e-12)
 
 
