@@ -261,8 +261,10 @@ int itcpd_plan_create(itcpd_plan** out, int32_t order, const int64_t* dims, int6
     for (int q = 0; q < order; ++q) {
         e = hipEventCreateWithFlags(&p->ev_main[q], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_side[q], hipEventDisableTiming);
-        if (e == hipSuccess && q < ITCPD_POLL_SLOTS)
-            e = hipEventCreateWithFlags(&p->ev_poll[q], hipEventDisableTiming);
+        if (e != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
+    }
+    for (int q = 0; q < ITCPD_POLL_SLOTS; ++q) {
+        e = hipEventCreateWithFlags(&p->ev_poll[q], hipEventDisableTiming);
         if (e != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
     }
 

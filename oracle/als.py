@@ -495,7 +495,8 @@ def optimize(T, cp: CPD, alg=None, check=None, verbose=False, trace: Optional[di
             st.gram_list[n] = factors[n].T @ factors[n]
         done = check_converge(check, factors, lam, st.gram_list, mtkrp, verbose)
         if isinstance(check, FitCheck):
-            st.fits.append(check.lastfit if not done else check.final_fit)
+            # this sweep's fit: after a reset (converged or max) it sits in final_fit
+            st.fits.append(check.final_fit if check.iter == 0 else check.lastfit)
         if done:
             break
         it += 1

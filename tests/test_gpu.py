@@ -64,12 +64,31 @@ SHAPES = [((12, 10, 8), 3), ((64, 64, 64), 10), ((33, 17, 29), 7), ((40, 30, 20)
 
 @pytest.mark.parametrize("dims,R", SHAPES, ids=[f"{'x'.join(map(str, d))}-R{r}" for d, r in SHAPES])
 def test_sweeps_match_oracle(itcpd, dims, R):
-    T, _ = lowrank(dims, min(R, 8), 1e-3, 1)
+    # a rank-R target decomposed at rank R: a well-posed problem, held to 1e-10
+    T, _ = lowrank(dims, R, 1e-3, 1)
     init = O.random_CPD(T, R, np.random.default_rng(2))
     got, _ = run_gpu(itcpd, T, init, itcpd.NoCheck(6))
     ref = O.als_optimize(T, init, check=O.NoCheck(6))
     assert factors_err(got, ref) < TOL
     assert relerr(got.lam, ref.lam) < TOL
+
+
+OVER = [((96, 80, 72), 64), ((50, 60, 70), 80), ((40, 30, 25), 150), ((20, 30, 40), 128)]
+
+
+@pytest.mark.parametrize("dims,R", OVER, ids=[f"{'x'.join(map(str, d))}-R{r}" for d, r in OVER])
+def test_overfactored_sweeps_within_rounding_spread(itcpd, dims, R):
+    """A rank-8 target decomposed at rank R >> 8: the Gram Hadamard is ill-conditioned
+    and two CPU evaluations of the same sweep (GEMM-form vs pairwise-contraction
+    MTTKRP, same solves) already differ by ~1e-9..1e-8 after 6 sweeps.  The GPU must
+    agree with the oracle within 10x that measured spread."""
+    T, _ = lowrank(dims, 8, 1e-3, 1)
+    init = O.random_CPD(T, R, np.random.default_rng(2))
+    got, _ = run_gpu(itcpd, T, init, itcpd.NoCheck(6))
+    ref = O.als_optimize(T, init, check=O.NoCheck(6), form="gemm")
+    alt = O.als_optimize(T, init, check=O.NoCheck(6), form="tree")
+    spread = factors_err(alt, ref)
+    assert factors_err(got, ref) < max(TOL, 10 * spread)
 
 
 def test_float32_tensor(itcpd):
@@ -230,9 +249,9 @@ def test_errors_are_status_codes(itcpd):
             plan.run(itcpd.NoCheck(3))
         assert e.value.code == L.ITCPD_E_STATE
         plan.random_cpd(1)
-        with pytest.raises(L.ITCPDError) as e:
-            plan.mttkrp(7)
-        assert e.value.code == L.ITCPD_E_ARG
+        buf = np.empty(64)
+        rc = plan._lib.itcpd_mttkrp(plan._h, 7, buf.ctypes.data, L.ITCPD_MEM_HOST)
+        assert rc == L.ITCPD_E_ARG and b"mode" in plan._lib.itcpd_last_error()
     with pytest.raises(L.ITCPDError) as e:
         itcpd.ALSPlan(T, 2, prec=L.ITCPD_BF16X3 + 5)
     assert e.value.code in (L.ITCPD_E_UNSUPPORTED, L.ITCPD_E_ARG)
