@@ -169,14 +169,30 @@ def _seed_of(rng) -> int:
 
 
 # ------------------------------------------------------------------------ plan
+_PRECS = {"fp64": L.ITCPD_FP64, "fp32": L.ITCPD_FP32, "bf16x3": L.ITCPD_BF16X3}
+
+
+def _prec_code(prec) -> int:
+    """MTTKRP arithmetic: "fp64" (the reference's), "fp32" or "bf16x3" (no reference
+    counterpart; the Gram, solve and fit stay FP64), or an ITCPD_* code."""
+    if isinstance(prec, str):
+        if prec.lower() not in _PRECS:
+            raise ValueError(f"prec must be one of {sorted(_PRECS)}")
+        return _PRECS[prec.lower()]
+    return int(prec)
+
+
 class ALSPlan:
     """A device-resident ALS problem (the ``ALS`` struct, als_optimizer.jl:5-10).
 
     Binds the target once; the CPD can be set, run and read back repeatedly.
+    ``prec`` selects the MTTKRP arithmetic ("fp64", "fp32", "bf16x3").
     """
 
-    def __init__(self, target, rank: int, alg=None, prec: int = L.ITCPD_FP64, device=None):
+    def __init__(self, target, rank: int, alg=None, prec="fp64", device=None):
         lib = L.load()
+        prec = _prec_code(prec)
+        self.prec = prec
         self._t = _Target(target)
         self.dims = self._t.dims
         self.rank = int(rank)
@@ -184,7 +200,7 @@ class ALSPlan:
         dims = (C.c_int64 * len(self.dims))(*self.dims)
         h = C.c_void_p()
         L.check(lib.itcpd_plan_create(C.byref(h), len(self.dims), dims, self.rank, self._t.elt,
-                                      C.c_void_p(self._t.ptr), self._t.mem, int(prec),
+                                      C.c_void_p(self._t.ptr), self._t.mem, prec,
                                       _alg_code(alg), dev))
         self._h = h
         self._lib = lib

@@ -220,8 +220,8 @@ int itcpd_plan_create(itcpd_plan** out, int32_t order, const int64_t* dims, int6
     if (elt_bytes != 8 && elt_bytes != 4) return fail(ITCPD_E_ARG, "elt_bytes must be 8 or 4");
     if (tensor_mem != ITCPD_MEM_HOST && tensor_mem != ITCPD_MEM_DEVICE)
         return fail(ITCPD_E_ARG, "tensor_mem must be ITCPD_MEM_HOST or ITCPD_MEM_DEVICE");
-    if (prec != ITCPD_FP64)
-        return fail(ITCPD_E_UNSUPPORTED, "only the FP64 MTTKRP is available in this build");
+    if (prec != ITCPD_FP64 && prec != ITCPD_FP32 && prec != ITCPD_BF16X3)
+        return fail(ITCPD_E_UNSUPPORTED, "prec must be ITCPD_FP64, ITCPD_FP32 or ITCPD_BF16X3");
     if (alg != ITCPD_ALG_KRP_FREE_NORMAL && alg != ITCPD_ALG_KRP_NORMAL)
         return fail(ITCPD_E_ARG, "alg must be KRPFreeNormal or KRPNormal");
     for (int q = 0; q < order; ++q)
@@ -287,6 +287,7 @@ int itcpd_plan_create(itcpd_plan** out, int32_t order, const int64_t* dims, int6
     if ((rc = dmalloc(&p->qr_aux, (size_t)(R * R + 4 * R)))) return bail(rc);
     if ((rc = dmalloc(&p->piv, (size_t)(2 * R)))) return bail(rc);
     if ((rc = dmalloc(&p->finfo, 4))) return bail(rc);
+    if (prec != ITCPD_FP64 && (rc = dmalloc(&p->Af, (size_t)(maxI * R)))) return bail(rc);
     if (p->s > 1 && (rc = dmalloc(&p->PL, (size_t)(p->ML * R)))) return bail(rc);
     if (order - p->s > 1 && (rc = dmalloc(&p->PR, (size_t)(p->MR * R)))) return bail(rc);
     {
@@ -332,6 +333,7 @@ int itcpd_plan_destroy(itcpd_plan* p) {
     dfree(p->qr_aux);
     dfree(p->piv);
     dfree(p->finfo);
+    dfree(p->Af);
     dfree(p->PL);
     dfree(p->PR);
     dfree(p->parts);

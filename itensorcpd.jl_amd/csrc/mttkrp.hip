@@ -310,6 +310,402 @@ __global__ __launch_bounds__(256, 2) void k_tr(const TT* __restrict__ T, int64_t
         }
 }
 
+// ------------------------------------------------- reduced-precision GEMMs
+// The same two GEMMs with FP32 arithmetic: either FP32 MFMA (v_mfma_f32_16x16x4f32)
+// or BF16x3 (v_mfma_f32_16x16x32_bf16 on hi*hi + hi*lo + lo*hi, where x = hi + lo
+// with hi = bf16(x), lo = bf16(x - hi)).  Accumulation is FP32 inside a wave;
+// the wave / split-K partials are combined and stored in FP64, so everything
+// downstream (Hadamard reductions, solve, Gram, fit) stays FP64.
+//
+// Both kernels walk the reduction in steps of 32 indices of the side's first
+// mode; lane group g = l >> 4 owns the 8 consecutive indices 8g .. 8g+7 of a step.
+// For BF16x3 those are the 8 k-values of one 16x16x32 MFMA (lane l holds
+// A[l & 15][8(l >> 4) + e] and B[8(l >> 4) + e][l & 15]); for FP32 they feed 8
+// 16x16x4 MFMAs (MFMA e takes k = 8g + e).  Results of both: lane l, register v
+// holds D[4 (l >> 4) + v][l & 15] (layouts measured on the device).
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi), for the 8 k-values of a lane.
+__device__ __forceinline__ void split8(const float (&x)[8], bf8& hi, bf8& lo) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        hi[e] = (__bf16)x[e];
+        lo[e] = (__bf16)(x[e] - (float)hi[e]);
+    }
+}
+
+template <typename TT>
+__device__ __forceinline__ float ldf(const TT* p) { return (float)p[0]; }
+
+// n consecutive elements starting at a 4n-byte (float) / 8n-byte (double) aligned address
+template <typename TT, int n>
+__device__ __forceinline__ void ld_runf(const TT* __restrict__ p, float* out) {
+    if constexpr (sizeof(TT) == 4 && n % 4 == 0) {
+        const f4* q = reinterpret_cast<const f4*>(p);
+#pragma unroll
+        for (int k = 0; k < n / 4; ++k) {
+            const f4 v = q[k];
+            out[4 * k] = v.x; out[4 * k + 1] = v.y; out[4 * k + 2] = v.z; out[4 * k + 3] = v.w;
+        }
+    } else if constexpr (sizeof(TT) == 4 && n == 2) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const f2 v = *reinterpret_cast<const f2*>(p);
+        out[0] = v.x; out[1] = v.y;
+    } else if constexpr (sizeof(TT) == 8 && n % 2 == 0) {
+        const dv2* q = reinterpret_cast<const dv2*>(p);
+#pragma unroll
+        for (int k = 0; k < n / 2; ++k) {
+            const dv2 v = q[k];
+            out[2 * k] = (float)v.x; out[2 * k + 1] = (float)v.y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < n; ++k) out[k] = (float)p[k];
+    }
+}
+
+// The factor operand of one step: A0(i, r) for the lane's 8 indices i = ib .. ib+7
+// at rank rk, from the pre-converted copy (FP32: one float array; BF16x3: the hi
+// array followed, `lo_off` elements later, by the lo array).
+template <bool BF>
+struct FacOp;
+template <>
+struct FacOp<false> {
+    float v[8];
+    template <bool FAST>
+    __device__ __forceinline__ void load(const void* base, int64_t lo_off, int64_t off, int64_t nvalid) {
+        const float* p = reinterpret_cast<const float*>(base) + off;
+        if constexpr (FAST) {
+            ld_runf<float, 8>(p, v);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = e < nvalid ? p[e] : 0.f;
+        }
+    }
+};
+template <>
+struct FacOp<true> {
+    bf8 hi, lo;
+    template <bool FAST>
+    __device__ __forceinline__ void load(const void* base, int64_t lo_off, int64_t off, int64_t nvalid) {
+        const __bf16* p = reinterpret_cast<const __bf16*>(base) + off;
+        if constexpr (FAST) {
+            hi = *reinterpret_cast<const bf8*>(p);
+            lo = *reinterpret_cast<const bf8*>(p + lo_off);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                hi[e] = e < nvalid ? p[e] : (__bf16)0.f;
+                lo[e] = e < nvalid ? p[lo_off + e] : (__bf16)0.f;
+            }
+        }
+    }
+};
+
+// acc(RB x MB tiles) += A * B over one step of 32 reduction indices; b[j][e] holds
+// B[8g+e][n] of column block j.  The tensor operand is split (BF16x3) once per step.
+template <bool BF, int RB, int MB>
+__device__ __forceinline__ void lp_step(const FacOp<BF> (&a)[RB], const float (&b)[MB][8],
+                                        f4 (&acc)[RB][MB]) {
+    if constexpr (BF) {
+#pragma unroll
+        for (int j = 0; j < MB; ++j) {
+            bf8 bh, bl;
+            split8(b[j], bh, bl);
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[q].lo, bh, acc[q][j], 0, 0, 0);
+                acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[q].hi, bl, acc[q][j], 0, 0, 0);
+                acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[q].hi, bh, acc[q][j], 0, 0, 0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+#pragma unroll
+            for (int q = 0; q < RB; ++q)
+#pragma unroll
+                for (int j = 0; j < MB; ++j)
+                    acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q].v[e], b[j][e], acc[q][j], 0, 0, 0);
+    }
+}
+
+// tot += w * acc for the accumulator layout (rank r0 + 16 a + 4 g + v), then acc = 0.
+// KRP(i + I0 rest, r) = A0(i, r) w_rest(r): the GEMM runs against A0 alone over one
+// value of `rest` and the weight is applied to that partial sum.
+template <int RB, int MB>
+__device__ __forceinline__ void fold_rest(const Side& sd, int64_t rest, int r0, int g, int R,
+                                          f4 (&acc)[RB][MB], f4 (&tot)[RB][MB]) {
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + 16 * a + 4 * g + v;
+            const float w = r < R ? (float)rest_weight(sd, rest, r, 0, false) : 0.f;
+#pragma unroll
+            for (int b = 0; b < MB; ++b) {
+                tot[a][b][v] += w * acc[a][b][v];
+                acc[a][b][v] = 0.f;
+            }
+        }
+}
+
+// Left GEMM, reduced precision.  out(mL, r) = sum_{mR} T(mL, mR) KRP(mR, r) with
+// mR = i + I0 rest.  Wave tile: 16*MB rows x 16*RB ranks; lane (g, c) owns rows
+// m0 + MB c .. + MB-1 of MFMA column c and reduction indices 8g .. 8g+7 of a step.
+template <typename TT, int RB, int MB, bool BF, bool FAST>
+__global__ __launch_bounds__(256, 2) void k_tk_lp(const TT* __restrict__ T, int64_t ML, Side sd,
+                                                  const void* __restrict__ A0c, int64_t lo_off,
+                                                  int64_t nsteps_i, int64_t kk_per_split,
+                                                  int64_t kk_total, int R,
+                                                  double* __restrict__ out, int64_t out_ld,
+                                                  int64_t split_stride,
+                                                  const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int64_t m0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * MB);
+    if (m0 >= ML) return;
+    const int r0 = blockIdx.y * (16 * RB);
+    const int64_t kk0 = (int64_t)blockIdx.z * kk_per_split;
+    const int64_t kk1 = kk0 + kk_per_split < kk_total ? kk0 + kk_per_split : kk_total;
+    const int64_t I0 = sd.dim[0];
+    const int64_t mrow = m0 + MB * c;
+    int64_t foff[RB];
+#pragma unroll
+    for (int a = 0; a < RB; ++a) {
+        const int r = r0 + 16 * a + c;
+        foff[a] = I0 * (int64_t)(r < R ? r : 0);   // rank >= R: its outputs are dropped
+    }
+
+    f4 acc[RB][MB], tot[RB][MB];
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+            tot[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+        }
+
+    int64_t kk = kk0;
+    while (kk < kk1) {
+        const int64_t rest = kk / nsteps_i;
+        const int64_t s_lo = kk - rest * nsteps_i;
+        int64_t s_hi = nsteps_i;
+        if ((rest + 1) * nsteps_i > kk1) s_hi = kk1 - rest * nsteps_i;
+        const TT* __restrict__ Tr = T + (I0 * rest) * ML + mrow;
+
+        float tv[2][MB][8];
+        FacOp<BF> kv[2][RB];
+        auto load = [&](int buf, int64_t st) {
+            const int64_t ib = st * 32 + 8 * g;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t i = ib + e;
+                float run[MB];
+                if constexpr (FAST) {
+                    ld_runf<TT, MB>(Tr + i * ML, run);
+                } else {
+#pragma unroll
+                    for (int b = 0; b < MB; ++b)
+                        run[b] = (i < I0 && mrow + b < ML) ? ldf(Tr + i * ML + b) : 0.f;
+                }
+#pragma unroll
+                for (int b = 0; b < MB; ++b) tv[buf][b][e] = run[b];
+            }
+#pragma unroll
+            for (int a = 0; a < RB; ++a) kv[buf][a].template load<FAST>(A0c, lo_off, foff[a] + ib, I0 - ib);
+        };
+        auto compute = [&](int buf) { lp_step<BF, RB, MB>(kv[buf], tv[buf], acc); };
+        int64_t st = s_lo;
+        load(0, st);
+        while (true) {
+            const bool more1 = st + 1 < s_hi;
+            if (more1) load(1, st + 1);
+            compute(0);
+            st += 1;
+            if (!more1) break;
+            const bool more0 = st + 1 < s_hi;
+            if (more0) load(0, st + 1);
+            compute(1);
+            st += 1;
+            if (!more0) break;
+        }
+        fold_rest<RB, MB>(sd, rest, r0, g, R, acc, tot);
+        kk = rest * nsteps_i + s_hi;
+    }
+
+    double* o = out + (int64_t)blockIdx.z * split_stride;
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + 16 * a + 4 * g + v;
+            if (r >= R) continue;
+#pragma unroll
+            for (int b = 0; b < MB; ++b) {
+                const int64_t row = mrow + b;
+                if (FAST || row < ML) o[row + out_ld * r] = (double)tot[a][b][v];
+            }
+        }
+}
+
+// Right GEMM, reduced precision.  out(mR, r) = sum_{mL} T(mL, mR) KRP(mL, r) with
+// mL = i + I0 rest.  Workgroup tile 16*MB columns x 16*RB ranks; the 4 waves split
+// the workgroup's reduction range and add their FP64 partials through LDS.  Lane
+// (g, c) reads 8 consecutive elements of each of its MB columns per step, so a
+// wave reads 128 contiguous bytes (FP32) of every column it touches.
+template <typename TT, int RB, int MB, bool BF, bool FAST>
+__global__ __launch_bounds__(256, 2) void k_tr_lp(const TT* __restrict__ T, int64_t ML, int64_t MR,
+                                                  Side sd, const void* __restrict__ A0c,
+                                                  int64_t lo_off, int64_t nsteps_i,
+                                                  int64_t kk_per_split, int64_t kk_total, int R,
+                                                  double* __restrict__ out, int64_t split_stride,
+                                                  const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    __shared__ double red[2][RB * MB * 4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int64_t n0 = (int64_t)blockIdx.x * (16 * MB);
+    const int r0 = blockIdx.y * (16 * RB);
+    const int64_t kb0 = (int64_t)blockIdx.z * kk_per_split;
+    const int64_t kb1 = kb0 + kk_per_split < kk_total ? kb0 + kk_per_split : kk_total;
+    const int64_t quarter = (kb1 - kb0 + 3) / 4;
+    const int64_t kk0 = kb0 + wave * quarter < kb1 ? kb0 + wave * quarter : kb1;
+    const int64_t kk1 = kk0 + quarter < kb1 ? kk0 + quarter : kb1;
+    const int64_t I0 = sd.dim[0];
+
+    int64_t col[MB];
+    bool colok[MB];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        col[b] = n0 + 16 * b + c;
+        colok[b] = FAST || col[b] < MR;
+        if (!colok[b]) col[b] = 0;
+    }
+    int64_t foff[RB];
+#pragma unroll
+    for (int a = 0; a < RB; ++a) {
+        const int r = r0 + 16 * a + c;
+        foff[a] = I0 * (int64_t)(r < R ? r : 0);
+    }
+
+    f4 acc[RB][MB], tot[RB][MB];
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+            tot[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+        }
+
+    int64_t kk = kk0;
+    while (kk < kk1) {
+        const int64_t rest = kk / nsteps_i;
+        const int64_t s_lo = kk - rest * nsteps_i;
+        int64_t s_hi = nsteps_i;
+        if ((rest + 1) * nsteps_i > kk1) s_hi = kk1 - rest * nsteps_i;
+        const TT* __restrict__ Tr = T + I0 * rest;
+
+        float tv[2][MB][8];
+        FacOp<BF> kv[2][RB];
+        auto load = [&](int buf, int64_t st) {
+            const int64_t ib = st * 32 + 8 * g;
+#pragma unroll
+            for (int b = 0; b < MB; ++b) {
+                const TT* tp = Tr + ib + ML * col[b];
+                if constexpr (FAST) {
+                    ld_runf<TT, 8>(tp, tv[buf][b]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) tv[buf][b][e] = (colok[b] && ib + e < I0) ? ldf(tp + e) : 0.f;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < RB; ++a) kv[buf][a].template load<FAST>(A0c, lo_off, foff[a] + ib, I0 - ib);
+        };
+        auto compute = [&](int buf) { lp_step<BF, RB, MB>(kv[buf], tv[buf], acc); };
+        int64_t st = s_lo;
+        load(0, st);
+        while (true) {
+            const bool more1 = st + 1 < s_hi;
+            if (more1) load(1, st + 1);
+            compute(0);
+            st += 1;
+            if (!more1) break;
+            const bool more0 = st + 1 < s_hi;
+            if (more0) load(0, st + 1);
+            compute(1);
+            st += 1;
+            if (!more0) break;
+        }
+        fold_rest<RB, MB>(sd, rest, r0, g, R, acc, tot);
+        kk = rest * nsteps_i + s_hi;
+    }
+
+    // the four waves' partials: (2,3) -> (0,1), then 1 -> 0
+    if (wave >= 2) {
+#pragma unroll
+        for (int a = 0; a < RB; ++a)
+#pragma unroll
+            for (int b = 0; b < MB; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) red[wave - 2][(a * MB + b) * 4 + v][lane] = (double)tot[a][b][v];
+    }
+    __syncthreads();
+    double sum[RB][MB][4];
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int b = 0; b < MB; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                sum[a][b][v] = (double)tot[a][b][v] + (wave < 2 ? red[wave & 1][(a * MB + b) * 4 + v][lane] : 0.0);
+    __syncthreads();
+    if (wave == 1) {
+#pragma unroll
+        for (int a = 0; a < RB; ++a)
+#pragma unroll
+            for (int b = 0; b < MB; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) red[0][(a * MB + b) * 4 + v][lane] = sum[a][b][v];
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    double* o = out + (int64_t)blockIdx.z * split_stride;
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + 16 * a + 4 * g + v;
+#pragma unroll
+            for (int b = 0; b < MB; ++b) {
+                const double val = sum[a][b][v] + red[0][(a * MB + b) * 4 + v][lane];
+                if (r < R && colok[b]) o[col[b] + MR * r] = val;
+            }
+        }
+}
+
+// Converted copy of a column-major factor (I x R, n = I R elements) for the
+// reduced-precision GEMMs: FP32, or BF16 hi (n elements) followed by BF16 lo.
+__global__ __launch_bounds__(256) void k_convert_factor(const double* __restrict__ A, int64_t n,
+                                                        int bf, void* __restrict__ out,
+                                                        const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float x = (float)A[i];
+    if (bf) {
+        __bf16* o = reinterpret_cast<__bf16*>(out);
+        const __bf16 hi = (__bf16)x;
+        o[i] = hi;
+        o[n + i] = (__bf16)(x - (float)hi);
+    } else {
+        reinterpret_cast<float*>(out)[i] = x;
+    }
+}
+
 // ------------------------------------------------------- Hadamard-reduce (ttv)
 // M(i, r) = sum_{a<A, b<B} P(a + A*i + A*In*b, r) * Wa(a, r) * Wb(b, r)
 // Wa = product over the side's modes before `pos`, Wb over the modes after it
@@ -569,6 +965,71 @@ int dispatch_tr(itcpd_plan* p, const Side& sd, double* out, int64_t nz, int64_t 
     }
 }
 
+template <typename TT, int RB, int MB, bool BF>
+int run_tk_lp(itcpd_plan* p, const Side& sd, double* out, int64_t out_ld, int64_t nz,
+              int64_t kk_per, int64_t kk_total, int64_t nsteps_i, int64_t split_stride) {
+    dim3 grid((unsigned)cdiv(p->ML, 64 * MB), (unsigned)(p->Rp / (16 * RB)), (unsigned)nz);
+    const bool fast = (p->ML % (64 * MB) == 0) && (sd.dim[0] % 32 == 0);
+    if (fast)
+        hipLaunchKernelGGL((k_tk_lp<TT, RB, MB, BF, true>), grid, dim3(256), 0, p->stream,
+                           (const TT*)p->T, p->ML, sd, (const void*)p->Af, sd.dim[0] * p->R,
+                           nsteps_i, kk_per, kk_total, (int)p->R, out, out_ld, split_stride, p->chk);
+    else
+        hipLaunchKernelGGL((k_tk_lp<TT, RB, MB, BF, false>), grid, dim3(256), 0, p->stream,
+                           (const TT*)p->T, p->ML, sd, (const void*)p->Af, sd.dim[0] * p->R,
+                           nsteps_i, kk_per, kk_total, (int)p->R, out, out_ld, split_stride, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+template <typename TT, int RB, int MB, bool BF>
+int run_tr_lp(itcpd_plan* p, const Side& sd, double* out, int64_t nz, int64_t kk_per,
+              int64_t kk_total, int64_t nsteps_i, int64_t split_stride) {
+    dim3 grid((unsigned)cdiv(p->MR, 16 * MB), (unsigned)(p->Rp / (16 * RB)), (unsigned)nz);
+    const bool fast = (p->MR % (16 * MB) == 0) && (sd.dim[0] % 32 == 0) && (p->ML % 4 == 0);
+    if (fast)
+        hipLaunchKernelGGL((k_tr_lp<TT, RB, MB, BF, true>), grid, dim3(256), 0, p->stream,
+                           (const TT*)p->T, p->ML, p->MR, sd, (const void*)p->Af, sd.dim[0] * p->R,
+                           nsteps_i, kk_per, kk_total, (int)p->R, out, split_stride, p->chk);
+    else
+        hipLaunchKernelGGL((k_tr_lp<TT, RB, MB, BF, false>), grid, dim3(256), 0, p->stream,
+                           (const TT*)p->T, p->ML, p->MR, sd, (const void*)p->Af, sd.dim[0] * p->R,
+                           nsteps_i, kk_per, kk_total, (int)p->R, out, split_stride, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+// Rank blocks (at most 4) and row / column blocks of the reduced-precision GEMMs,
+// sized so the double-buffered operands and accumulators fit 256 VGPRs.
+int rb_lp(const itcpd_plan* p) { return p->RB < 4 ? p->RB : 4; }
+constexpr int mb_lp(int RB, bool, int) { return RB <= 2 ? 4 : 2; }
+
+template <typename TT, bool BF>
+int dispatch_tk_lp(itcpd_plan* p, const Side& sd, double* out, int64_t out_ld, int64_t nz,
+                   int64_t kk_per, int64_t kk_total, int64_t nsteps_i, int64_t split_stride) {
+    switch (rb_lp(p)) {
+        case 1: return run_tk_lp<TT, 1, mb_lp(1, BF, sizeof(TT)), BF>(p, sd, out, out_ld, nz, kk_per, kk_total, nsteps_i, split_stride);
+        case 2: return run_tk_lp<TT, 2, mb_lp(2, BF, sizeof(TT)), BF>(p, sd, out, out_ld, nz, kk_per, kk_total, nsteps_i, split_stride);
+        default: return run_tk_lp<TT, 4, mb_lp(4, BF, sizeof(TT)), BF>(p, sd, out, out_ld, nz, kk_per, kk_total, nsteps_i, split_stride);
+    }
+}
+template <typename TT, bool BF>
+int dispatch_tr_lp(itcpd_plan* p, const Side& sd, double* out, int64_t nz, int64_t kk_per,
+                   int64_t kk_total, int64_t nsteps_i, int64_t split_stride) {
+    switch (rb_lp(p)) {
+        case 1: return run_tr_lp<TT, 1, mb_lp(1, BF, sizeof(TT)), BF>(p, sd, out, nz, kk_per, kk_total, nsteps_i, split_stride);
+        case 2: return run_tr_lp<TT, 2, mb_lp(2, BF, sizeof(TT)), BF>(p, sd, out, nz, kk_per, kk_total, nsteps_i, split_stride);
+        default: return run_tr_lp<TT, 4, mb_lp(4, BF, sizeof(TT)), BF>(p, sd, out, nz, kk_per, kk_total, nsteps_i, split_stride);
+    }
+}
+
+int convert_factor(itcpd_plan* p, const double* A, int64_t n) {
+    hipLaunchKernelGGL(k_convert_factor, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, p->stream, A, n,
+                       p->prec == ITCPD_BF16X3 ? 1 : 0, (void*)p->Af, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
 int mb_of(int RB) { return RB <= 2 ? 8 : RB == 4 ? 4 : 2; }
 
 }  // namespace
@@ -603,7 +1064,11 @@ int finish_parts(itcpd_plan* p, double* parts, int64_t nz, int64_t n, double* ds
 
 // Left GEMM: rows = left multi-index (ML), reduction = right side.  Produces PL
 // (s > 1) or the M of mode 0 (s == 1).
+int launch_left_gemm_lp(itcpd_plan* p, MSrc* ms);
+int launch_right_gemm_lp(itcpd_plan* p, MSrc* ms);
+
 int launch_left_gemm(itcpd_plan* p, MSrc* ms) {
+    if (p->prec != ITCPD_FP64) return launch_left_gemm_lp(p, ms);
     const Side sd = make_side(p, p->s, p->N - p->s, /*rank_fast=*/true);
     const int MB = mb_of(p->RB);
     const int64_t nsteps_i = cdiv(sd.dim[0], 4);
@@ -633,6 +1098,7 @@ int launch_left_gemm(itcpd_plan* p, MSrc* ms) {
 // Right GEMM: rows = right multi-index (MR), reduction = left side.  Produces PR
 // (N-s > 1) or the M of mode N-1 (N-s == 1).
 int launch_right_gemm(itcpd_plan* p, MSrc* ms) {
+    if (p->prec != ITCPD_FP64) return launch_right_gemm_lp(p, ms);
     const Side sd = make_side(p, 0, p->s, /*rank_fast=*/false);
     const int MB = mb_of(p->RB);
     constexpr int S = TR_RUN;
@@ -655,6 +1121,72 @@ int launch_right_gemm(itcpd_plan* p, MSrc* ms) {
         rc = (p->elt_bytes == 8)
                  ? dispatch_tr<double>(p, sd, out, nz, kk_per, kk_total, ngroups, out_elems)
                  : dispatch_tr<float>(p, sd, out, nz, kk_per, kk_total, ngroups, out_elems);
+    }
+    if (rc) return rc;
+    return finish_parts(p, p->parts, nz, out_elems, dst, ms, p->N - p->s == 1);
+}
+
+
+// Reduced-precision forms of the two GEMMs (FP32 or BF16x3 arithmetic, FP64 output).
+int launch_left_gemm_lp(itcpd_plan* p, MSrc* ms) {
+    const Side sd = make_side(p, p->s, p->N - p->s, /*rank_fast=*/false);
+    const int RB = rb_lp(p), MB = mb_lp(RB, p->prec == ITCPD_BF16X3, p->elt_bytes);
+    int rc;
+    if ((rc = convert_factor(p, p->A[p->s], sd.dim[0] * p->R))) return rc;
+    const int64_t nsteps_i = cdiv(sd.dim[0], 32);
+    const int64_t nrest = p->MR / sd.dim[0];
+    const int64_t kk_total = nsteps_i * nrest;
+    const int64_t base = cdiv(p->ML, 64 * MB) * (p->Rp / (16 * RB));
+    int64_t nz = pick_split(base, kk_total, 1024, 4);
+    const int64_t out_elems = p->ML * p->R;
+    if (nz > 1 && (size_t)(nz * out_elems) > p->parts_elems) nz = (int64_t)(p->parts_elems / out_elems);
+    if (nz < 1) nz = 1;
+    const int64_t kk_per = cdiv(kk_total, nz);
+    nz = cdiv(kk_total, kk_per);
+    double* dst = (p->s > 1) ? p->PL : p->M;
+    double* out = nz > 1 ? p->parts : dst;
+    const bool bf = p->prec == ITCPD_BF16X3;
+    {
+        Timed tm(p, 0, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
+                 (double)p->ML * (double)p->MR * p->elt_bytes);
+        if (p->elt_bytes == 8)
+            rc = bf ? dispatch_tk_lp<double, true>(p, sd, out, p->ML, nz, kk_per, kk_total, nsteps_i, out_elems)
+                    : dispatch_tk_lp<double, false>(p, sd, out, p->ML, nz, kk_per, kk_total, nsteps_i, out_elems);
+        else
+            rc = bf ? dispatch_tk_lp<float, true>(p, sd, out, p->ML, nz, kk_per, kk_total, nsteps_i, out_elems)
+                    : dispatch_tk_lp<float, false>(p, sd, out, p->ML, nz, kk_per, kk_total, nsteps_i, out_elems);
+    }
+    if (rc) return rc;
+    return finish_parts(p, p->parts, nz, out_elems, dst, ms, p->s == 1);
+}
+
+int launch_right_gemm_lp(itcpd_plan* p, MSrc* ms) {
+    const Side sd = make_side(p, 0, p->s, /*rank_fast=*/false);
+    const int RB = rb_lp(p), MB = mb_lp(RB, p->prec == ITCPD_BF16X3, p->elt_bytes);
+    int rc;
+    if ((rc = convert_factor(p, p->A[0], sd.dim[0] * p->R))) return rc;
+    const int64_t nsteps_i = cdiv(sd.dim[0], 32);
+    const int64_t nrest = p->ML / sd.dim[0];
+    const int64_t kk_total = nsteps_i * nrest;
+    const int64_t base = cdiv(p->MR, 16 * MB) * (p->Rp / (16 * RB));
+    int64_t nz = pick_split(base, kk_total, 512, 8);
+    const int64_t out_elems = p->MR * p->R;
+    if (nz > 1 && (size_t)(nz * out_elems) > p->parts_elems) nz = (int64_t)(p->parts_elems / out_elems);
+    if (nz < 1) nz = 1;
+    const int64_t kk_per = cdiv(kk_total, nz);
+    nz = cdiv(kk_total, kk_per);
+    double* dst = (p->N - p->s > 1) ? p->PR : p->M;
+    double* out = nz > 1 ? p->parts : dst;
+    const bool bf = p->prec == ITCPD_BF16X3;
+    {
+        Timed tm(p, 1, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
+                 (double)p->ML * (double)p->MR * p->elt_bytes);
+        if (p->elt_bytes == 8)
+            rc = bf ? dispatch_tr_lp<double, true>(p, sd, out, nz, kk_per, kk_total, nsteps_i, out_elems)
+                    : dispatch_tr_lp<double, false>(p, sd, out, nz, kk_per, kk_total, nsteps_i, out_elems);
+        else
+            rc = bf ? dispatch_tr_lp<float, true>(p, sd, out, nz, kk_per, kk_total, nsteps_i, out_elems)
+                    : dispatch_tr_lp<float, false>(p, sd, out, nz, kk_per, kk_total, nsteps_i, out_elems);
     }
     if (rc) return rc;
     return finish_parts(p, p->parts, nz, out_elems, dst, ms, p->N - p->s == 1);

@@ -1,0 +1,140 @@
+"""Reduced-precision MTTKRP forms (FP32 MFMA, BF16x3 MFMA; Gram, solve and fit in FP64)
+against the FP64 CPU oracle and the plain FP64 GPU path.
+
+There is no reference counterpart: the reference computes only in the element type
+of the target.  The bars below are the accuracy each form is built to deliver:
+
+* one MTTKRP: FP32 inputs rounded to 24 bits and FP32 accumulation give a relative
+  Frobenius error near 1e-7 (bar 1e-6).  BF16x3 keeps about 16 bits of each operand
+  (x = hi + lo, two bfloat16 terms) and drops the lo*lo product, so each product
+  carries a relative error up to ~2^-16: measured ~4.5e-6 (bar 2e-5).
+* a whole ALS run on an ill-conditioned (collinear) problem, 40 sweeps from the same
+  init: the final fit agrees with the FP64 run's fit to 1e-6 (FP32) / 5e-5 (BF16x3;
+  measured up to 1.7e-5 on 64^3 rank 24, where the iterates have not converged and
+  the two trajectories drift apart).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BARS = {"fp32": 1e-6, "bf16x3": 2e-5}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def collinear(dims, R, c, noise, seed, dtype=np.float32):
+    """Factors with exact pairwise column cosine c: A_n = Q_n L^T, L = chol((1-c) I + c 11^T)."""
+    rng = np.random.default_rng(seed)
+    L = np.linalg.cholesky((1 - c) * np.eye(R) + c * np.ones((R, R)))
+    fac = []
+    for d in dims:
+        Q, _ = np.linalg.qr(rng.standard_normal((d, R)))
+        fac.append(Q @ L.T)
+    T = O.reconstruct(fac, np.ones(R)) + noise * rng.standard_normal(dims)
+    return T.astype(dtype)
+
+
+HOOK_CASES = [((64, 64, 64), 32, np.float32), ((32, 32, 32, 32), 64, np.float32),
+              ((20, 17, 13, 9), 37, np.float32), ((64, 48, 40), 20, np.float64),
+              ((96, 64, 32), 128, np.float32), ((128, 128, 128), 8, np.float32),
+              ((33, 40, 7), 3, np.float32)]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("dims,R,dt", HOOK_CASES,
+                         ids=[f"{'x'.join(map(str, d))}-R{r}-{np.dtype(t).name}" for d, r, t in HOOK_CASES])
+def test_mttkrp_error(itcpd, prec, dims, R, dt):
+    rng = np.random.default_rng(1)
+    T = rng.standard_normal(dims).astype(dt)
+    init = O.random_CPD(T, R, np.random.default_rng(2))
+    T64 = T.astype(np.float64)
+    with itcpd.ALSPlan(T, R, prec=prec) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        for n in range(len(dims)):
+            err = rel(plan.mttkrp(n), O.mttkrp(T64, init.factors, n))
+            assert err < BARS[prec], (n, err)
+
+
+@pytest.mark.parametrize("prec,fit_bar", [("fp32", 1e-6), ("bf16x3", 5e-5)])
+@pytest.mark.parametrize("dims,R", [((48, 48, 48, 48), 32), ((64, 64, 64), 24)])
+def test_collinear_als_fit(itcpd, prec, fit_bar, dims, R):
+    T = collinear(dims, R, 0.9, 1e-3, 3)
+    init = O.random_CPD(T, R, np.random.default_rng(4))
+    T64 = T.astype(np.float64)
+    nrm = float(np.linalg.norm(T64))
+    cc = O.FitCheck(0.0, 40, nrm)
+    ref = O.als_optimize(T64, init, check=cc)
+    cg = itcpd.FitCheck(0.0, 40, nrm)
+    with itcpd.ALSPlan(T, R, prec=prec) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        plan.run(cg)
+        got = plan.get_cpd()
+    # the fit of the returned CPD evaluated in FP64 (explicit residual) is within the
+    # bar of the FP64 run's fit; so is the fit the reduced-precision run reported (its
+    # <T, X> comes from the last mode's reduced-precision MTTKRP)
+    fit_true = 1 - np.linalg.norm(O.reconstruct(got.factors, got.lam) - T64) / nrm
+    assert abs(fit_true - cc.final_fit) < fit_bar
+    assert abs(cg.final_fit - cc.final_fit) < fit_bar
+    assert max(rel(a, b) for a, b in zip(got.factors, ref.factors)) < 3e-2
+
+
+def test_lp_plan_state_and_hooks(itcpd):
+    """The reduced-precision plan shares the FP64 solve / Gram / fit kernels: the R x R
+    solve hook is exact (to FP64 rounding) whatever the MTTKRP precision."""
+    T = collinear((24, 20, 16), 6, 0.5, 1e-3, 5)
+    init = O.random_CPD(T, 6, np.random.default_rng(6))
+    G = O.grams(init.factors)
+    M = O.mttkrp(T.astype(np.float64), init.factors, 1)
+    with itcpd.ALSPlan(T, 6, prec="bf16x3") as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        X, fb = plan.solve_ls(1, M)
+        assert np.abs(plan.compute_krp(1) - O.compute_krp(G, 1)).max() < 1e-14
+    Xc, fbc = O.solve_ls_problem(O.compute_krp(G, 1), M)
+    assert fb == fbc and rel(X, Xc) < 1e-12
+    with pytest.raises(ValueError):
+        itcpd.ALSPlan(T, 6, prec="fp16")
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_full_size_224_order4(itcpd, prec, c3_tensor):
+    """The precision workload at its real size: 224^4 FP32-stored (10.07 GB), rank 128,
+    collinear factors (cosine 0.9) + N(0, 1e-3).  Size-independent checks: the MTTKRP of
+    mode 0 against the plain FP64 GPU path, and 3 sweeps from the same init reaching
+    the same fit."""
+    T, init = c3_tensor
+    R = init.rank
+    res = {}
+    for p in ("fp64", prec):
+        with itcpd.ALSPlan(T, R, prec=p) as plan:
+            plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+            M0 = plan.mttkrp(0)
+            nrm = plan.tensor_norm()
+            chk = itcpd.FitCheck(0.0, 3, nrm)
+            plan.run(chk)
+            res[p] = (M0, chk.final_fit)
+    assert rel(res[prec][0], res["fp64"][0]) < BARS[prec]
+    assert abs(res[prec][1] - res["fp64"][1]) < 1e-5
+
+
+@pytest.fixture(scope="module")
+def c3_tensor():
+    I, R = 224, 128
+    rng = np.random.default_rng(2025)
+    L = np.linalg.cholesky(0.1 * np.eye(R) + 0.9 * np.ones((R, R)))
+    fac = []
+    for _ in range(4):
+        Q, _ = np.linalg.qr(rng.standard_normal((I, R)))
+        fac.append(Q @ L.T)
+    K01 = (fac[1][:, None, :] * fac[0][None, :, :]).reshape(I * I, R).astype(np.float32)
+    K23 = (fac[3][:, None, :] * fac[2][None, :, :]).reshape(I * I, R).astype(np.float32)
+    T = K23 @ K01.T                # C-order rows k + I l, columns i + I j: flat = i + I j + I^2 (k + I l)
+    for r0 in range(0, I * I, 2048):
+        T[r0:r0 + 2048] += 1e-3 * rng.standard_normal((min(2048, I * I - r0), I * I), dtype=np.float32)
+    T = T.reshape(-1).reshape((I, I, I, I), order="F")
+    init = O.random_CPD(T, R, np.random.default_rng(5))
+    yield T, init
