@@ -156,6 +156,31 @@ int itcpd_plan_set_profiling(itcpd_plan* plan, int32_t on);
 int itcpd_get_kernel_stats(itcpd_plan* plan, int32_t kernel, itcpd_kernel_stats* stats);
 int itcpd_synchronize(itcpd_plan* plan);
 
+/* ---- batches of small problems ----
+ * No counterpart in the reference, where a caller loops decompose / als_optimize
+ * (decompose.jl:13-30, als_optimizer.jl:15-25) over its tensors.  Here many small
+ * order-3 Float64 problems (every dimension <= 96, rank <= 32) run `sweeps` ALS
+ * sweeps each (NoCheck semantics, no_check.jl:1-20) in one persistent kernel
+ * replayed from a captured HIP graph.
+ * dims: 3 per problem; tensors: the column-major tensors concatenated; factors:
+ * per problem A_0 | A_1 | A_2 (each I_n x R column-major), concatenated; lambdas:
+ * the R_b lambdas concatenated.  A device tensor array is borrowed. */
+typedef struct itcpd_batch itcpd_batch;
+int itcpd_batch_create(itcpd_batch** batch, int32_t nprob, const int64_t* dims, const int32_t* ranks,
+                       const double* tensors, int32_t tensor_mem, int32_t device);
+int itcpd_batch_destroy(itcpd_batch* batch);
+int itcpd_batch_set_cpd(itcpd_batch* batch, const double* factors, int32_t mem);
+int itcpd_batch_get_cpd(itcpd_batch* batch, double* factors, double* lambdas, int32_t mem);
+/* Runs `sweeps` sweeps of every problem from the current CPD; fits (host, nprob,
+ * nullable) receives each problem's FP64 fit after the last sweep, elapsed_ms
+ * (nullable) the device time of the launch. */
+int itcpd_batch_run(itcpd_batch* batch, int32_t sweeps, double* fits, double* elapsed_ms);
+/* Workgroup size of the persistent kernel: 256 or 512 threads. */
+int itcpd_batch_set_threads(itcpd_batch* batch, int32_t threads);
+/* Times `sweeps` sweeps with each workgroup size (restoring the CPD after each),
+ * keeps the fastest; ms_per_config (nullable) receives the 2 timings (256, 512). */
+int itcpd_batch_tune(itcpd_batch* batch, int32_t sweeps, int32_t* threads_chosen, double* ms_per_config);
+
 /* Thread-local text of the last failing call on this thread ("" if none). */
 const char* itcpd_last_error(void);
 

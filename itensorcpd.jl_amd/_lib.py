@@ -62,6 +62,14 @@ SIGNATURES = {
     "itcpd_plan_set_profiling": (C.c_int, [_P, C.c_int32]),
     "itcpd_get_kernel_stats": (C.c_int, [_P, C.c_int32, C.POINTER(KernelStats)]),
     "itcpd_synchronize": (C.c_int, [_P]),
+    "itcpd_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, _I64P, C.POINTER(C.c_int32), _P,
+                                     C.c_int32, C.c_int32]),
+    "itcpd_batch_destroy": (C.c_int, [_P]),
+    "itcpd_batch_set_cpd": (C.c_int, [_P, _P, C.c_int32]),
+    "itcpd_batch_get_cpd": (C.c_int, [_P, _P, _P, C.c_int32]),
+    "itcpd_batch_run": (C.c_int, [_P, C.c_int32, _DP, _DP]),
+    "itcpd_batch_set_threads": (C.c_int, [_P, C.c_int32]),
+    "itcpd_batch_tune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _DP]),
     "itcpd_last_error": (C.c_char_p, []),
     "itcpd_version": (C.c_char_p, []),
 }

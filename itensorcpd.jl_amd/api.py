@@ -449,3 +449,114 @@ def decompose(A, *args, solver=None, rng=None, alg=None, check=None, maxiter=Non
     if solver is None:
         return als_optimize(A, cp, alg=alg, check=check, maxiter=maxiter, verbose=verbose)
     raise RuntimeError("OptimizerError")
+
+
+# ------------------------------------------------------------------- batches
+class BatchALS:
+    """Many small order-3 Float64 problems (every dimension <= 96, rank <= 32), each
+    run for a fixed number of ALS sweeps (NoCheck semantics, no_check.jl:1-20) in one
+    persistent kernel replayed from a captured HIP graph.  No reference counterpart:
+    the reference would loop ``als_optimize`` (als_optimizer.jl:15-25) over them.
+    """
+
+    def __init__(self, tensors, ranks, device: int = 0):
+        lib = L.load()
+        self.tensors = [np.asfortranarray(t, dtype=np.float64) for t in tensors]
+        self.ranks = [int(r) for r in ranks]
+        if len(self.tensors) != len(self.ranks) or not self.tensors:
+            raise ValueError("one rank per tensor, at least one tensor")
+        for t in self.tensors:
+            if t.ndim != 3:
+                raise ValueError("batched problems are order-3 tensors")
+        self.dims = [tuple(int(d) for d in t.shape) for t in self.tensors]
+        flat = np.concatenate([t.ravel(order="F") for t in self.tensors])
+        dims = np.ascontiguousarray(np.array(self.dims, dtype=np.int64).ravel())
+        rk = np.ascontiguousarray(np.array(self.ranks, dtype=np.int32))
+        h = C.c_void_p()
+        L.check(lib.itcpd_batch_create(C.byref(h), len(self.tensors),
+                                       dims.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       rk.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       C.c_void_p(flat.ctypes.data), L.ITCPD_MEM_HOST, int(device)))
+        self._h = h
+        self._lib = lib
+        self.nprob = len(self.tensors)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.itcpd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_cpd(self, cps):
+        if len(cps) != self.nprob:
+            raise ValueError("one CPD per problem")
+        parts = []
+        for cp, d, r in zip(cps, self.dims, self.ranks):
+            if cp.rank != r or len(cp.factors) != 3:
+                raise ValueError("CPD does not match the problem's rank / order")
+            for f, n in zip(cp.factors, d):
+                if f.shape != (n, r):
+                    raise ValueError(f"factor shape {f.shape} != {(n, r)}")
+                parts.append(np.asarray(f, dtype=np.float64).ravel(order="F"))
+        flat = np.ascontiguousarray(np.concatenate(parts))
+        L.check(self._lib.itcpd_batch_set_cpd(self._h, C.c_void_p(flat.ctypes.data), L.ITCPD_MEM_HOST))
+
+    def get_cpd(self):
+        nf = sum((sum(d) * r) for d, r in zip(self.dims, self.ranks))
+        flat = np.empty(nf)
+        lam = np.empty(sum(self.ranks))
+        L.check(self._lib.itcpd_batch_get_cpd(self._h, C.c_void_p(flat.ctypes.data),
+                                              C.c_void_p(lam.ctypes.data), L.ITCPD_MEM_HOST))
+        out, fo, lo = [], 0, 0
+        for d, r in zip(self.dims, self.ranks):
+            fs = []
+            for n in d:
+                fs.append(flat[fo:fo + n * r].reshape((n, r), order="F").copy(order="F"))
+                fo += n * r
+            out.append(CPD(fs, lam[lo:lo + r].copy()))
+            lo += r
+        return out
+
+    def run(self, sweeps: int):
+        """Runs `sweeps` sweeps of every problem; returns (fits, device milliseconds)."""
+        fits = np.empty(self.nprob)
+        ms = C.c_double()
+        L.check(self._lib.itcpd_batch_run(self._h, int(sweeps),
+                                          fits.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ms)))
+        return fits, ms.value
+
+    def set_threads(self, threads: int):
+        L.check(self._lib.itcpd_batch_set_threads(self._h, int(threads)))
+
+    def tune(self, sweeps: int = 2):
+        """Times each workgroup size on `sweeps` sweeps (the CPD is restored) and keeps
+        the fastest; returns (threads, {threads: ms})."""
+        ch = C.c_int32()
+        ms = (C.c_double * 2)()
+        L.check(self._lib.itcpd_batch_tune(self._h, int(sweeps), C.byref(ch), ms))
+        return ch.value, {256: ms[0], 512: ms[1]}
+
+
+def decompose_batch(tensors, ranks, sweeps: int = 100, rng=None, tune: bool = True):
+    """decompose (decompose.jl:13-30) of every tensor with NoCheck(sweeps), in one
+    batched launch; each problem starts from random_CPD (seed from `rng`, offset by
+    its position).  Returns (list of CPD, FP64 fits)."""
+    base = _seed_of(rng)
+    inits = [random_CPD(t.shape, r, rng=base + 7919 * b) for b, (t, r) in enumerate(zip(tensors, ranks))]
+    with BatchALS(tensors, ranks) as bt:
+        bt.set_cpd(inits)
+        if tune:
+            bt.tune(2)
+        fits, _ = bt.run(sweeps)
+        return bt.get_cpd(), fits

@@ -1,7 +1,9 @@
 // C ABI of libitcpd_hip.so (declared in include/itcpd.h).
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "internal.h"
 
@@ -646,6 +648,233 @@ int itcpd_synchronize(itcpd_plan* p) {
     if (!p) return fail(ITCPD_E_ARG, "null plan");
     DeviceGuard dg(p->device);
     ITCPD_HIP(hipStreamSynchronize(p->stream));
+    return ITCPD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+struct itcpd_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    itcpd::BatchDev d{};
+    itcpd::BatchProblem* probs = nullptr;   // device
+    int* order = nullptr;                   // device
+    double* T = nullptr;                    // device (owned copy or borrowed)
+    bool T_owned = false;
+    double* factors = nullptr;
+    double* backup = nullptr;               // factors saved across tuning runs
+    double* lambdas = nullptr;
+    double* fits = nullptr;
+    int* counter = nullptr;
+    int64_t nfac = 0, nlam = 0, nel = 0;
+    int nprob = 0;
+    int nt = 256;
+    bool have_cpd = false;
+    // one instantiated graph (reset + persistent kernel) for the last (nt, sweeps)
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int g_nt = 0, g_sweeps = -1;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace itcpd {
+namespace {
+
+void batch_free(itcpd_batch* b) {
+    if (!b) return;
+    DeviceGuard dg(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    dfree(b->probs);
+    dfree(b->order);
+    if (b->T_owned) dfree(b->T);
+    dfree(b->factors);
+    dfree(b->backup);
+    dfree(b->lambdas);
+    dfree(b->fits);
+    dfree(b->counter);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// The graph replays reset + persistent kernel; rebuilt when (nt, sweeps) changes.
+int batch_graph(itcpd_batch* b, int sweeps) {
+    if (b->gexec && b->g_nt == b->nt && b->g_sweeps == sweeps) return ITCPD_OK;
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    ITCPD_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = launch_batch(b->d, b->nt, sweeps, b->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = g;
+    ITCPD_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->g_nt = b->nt;
+    b->g_sweeps = sweeps;
+    return ITCPD_OK;
+}
+
+int batch_launch_timed(itcpd_batch* b, int sweeps, double* ms) {
+    int rc;
+    if ((rc = batch_graph(b, sweeps))) return rc;
+    ITCPD_HIP(hipEventRecord(b->ev0, b->stream));
+    ITCPD_HIP(hipGraphLaunch(b->gexec, b->stream));
+    ITCPD_HIP(hipEventRecord(b->ev1, b->stream));
+    ITCPD_HIP(hipEventSynchronize(b->ev1));
+    float f = 0.f;
+    ITCPD_HIP(hipEventElapsedTime(&f, b->ev0, b->ev1));
+    if (ms) *ms = f;
+    return ITCPD_OK;
+}
+
+}  // namespace
+}  // namespace itcpd
+
+extern "C" {
+
+int itcpd_batch_create(itcpd_batch** out, int32_t nprob, const int64_t* dims, const int32_t* ranks,
+                       const double* tensors, int32_t tensor_mem, int32_t device) {
+    if (!out) return fail(ITCPD_E_ARG, "batch pointer is null");
+    *out = nullptr;
+    if (nprob < 1 || !dims || !ranks || !tensors) return fail(ITCPD_E_ARG, "empty batch or null input");
+    if (tensor_mem != ITCPD_MEM_HOST && tensor_mem != ITCPD_MEM_DEVICE)
+        return fail(ITCPD_E_ARG, "tensor_mem must be ITCPD_MEM_HOST or ITCPD_MEM_DEVICE");
+    std::vector<BatchProblem> hp((size_t)nprob);
+    int64_t toff = 0, foff = 0, loff = 0;
+    for (int b = 0; b < nprob; ++b) {
+        const int64_t I = dims[3 * b], J = dims[3 * b + 1], K = dims[3 * b + 2];
+        const int R = ranks[b];
+        if (I < 1 || J < 1 || K < 1 || I > BATCH_MAXD || J > BATCH_MAXD || K > BATCH_MAXD)
+            return fail(ITCPD_E_UNSUPPORTED, "batched problems need every dimension in [1, 96]");
+        if (R < 1 || R > BATCH_MAXR) return fail(ITCPD_E_UNSUPPORTED, "batched problems need rank in [1, 32]");
+        hp[b] = BatchProblem{toff, foff, (int)I, (int)J, (int)K, R, (int)loff, b};
+        toff += I * J * K;
+        foff += (I + J + K) * R;
+        loff += R;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(ITCPD_E_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(ITCPD_E_ARG, "device index out of range");
+    itcpd_batch* b = new (std::nothrow) itcpd_batch();
+    if (!b) return fail(ITCPD_E_OOM, "host allocation failed");
+    DeviceGuard dg(device);
+    b->device = device;
+    b->nprob = nprob;
+    b->nel = toff;
+    b->nfac = foff;
+    b->nlam = loff;
+    auto bail = [&](int code) { batch_free(b); return code; };
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+    if ((e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess)
+        return bail(hip_fail(e, "hipEventCreate"));
+    int rc;
+    // queue order: most work first (longest-processing-time order)
+    std::vector<int> ord((size_t)nprob);
+    for (int q = 0; q < nprob; ++q) ord[q] = q;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+        const double wx = (double)hp[x].I * hp[x].J * hp[x].K * hp[x].R;
+        const double wy = (double)hp[y].I * hp[y].J * hp[y].K * hp[y].R;
+        return wx > wy;
+    });
+    if ((rc = dmalloc(&b->probs, (size_t)nprob))) return bail(rc);
+    if ((rc = dmalloc(&b->order, (size_t)nprob))) return bail(rc);
+    if ((rc = dmalloc(&b->factors, (size_t)foff))) return bail(rc);
+    if ((rc = dmalloc(&b->backup, (size_t)foff))) return bail(rc);
+    if ((rc = dmalloc(&b->lambdas, (size_t)loff))) return bail(rc);
+    if ((rc = dmalloc(&b->fits, (size_t)nprob))) return bail(rc);
+    if ((rc = dmalloc(&b->counter, 1))) return bail(rc);
+    e = hipMemcpy(b->probs, hp.data(), sizeof(BatchProblem) * nprob, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->order, ord.data(), sizeof(int) * nprob, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy(batch descriptors)"));
+    if (tensor_mem == ITCPD_MEM_HOST) {
+        if ((rc = dmalloc(&b->T, (size_t)toff))) return bail(rc);
+        b->T_owned = true;
+        e = hipMemcpy(b->T, tensors, (size_t)toff * 8, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy(tensors)"));
+    } else {
+        e = hipDeviceSynchronize();
+        if (e != hipSuccess) return bail(hip_fail(e, "hipDeviceSynchronize"));
+        b->T = const_cast<double*>(tensors);
+    }
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipGetDeviceProperties"));
+    b->d = BatchDev{b->probs, b->order, nprob, b->T, b->factors, b->lambdas, b->fits, b->counter,
+                    std::min(nprob, prop.multiProcessorCount)};
+    *out = b;
+    return ITCPD_OK;
+}
+
+int itcpd_batch_destroy(itcpd_batch* b) {
+    batch_free(b);
+    return ITCPD_OK;
+}
+
+int itcpd_batch_set_cpd(itcpd_batch* b, const double* factors, int32_t mem) {
+    if (!b || !factors) return fail(ITCPD_E_ARG, "null argument");
+    DeviceGuard dg(b->device);
+    int rc;
+    if ((rc = copy_in(b->factors, factors, (size_t)b->nfac * 8, mem, b->stream))) return rc;
+    ITCPD_HIP(hipStreamSynchronize(b->stream));
+    b->have_cpd = true;
+    return ITCPD_OK;
+}
+
+int itcpd_batch_get_cpd(itcpd_batch* b, double* factors, double* lambdas, int32_t mem) {
+    if (!b) return fail(ITCPD_E_ARG, "null batch");
+    DeviceGuard dg(b->device);
+    int rc;
+    if (factors && (rc = copy_out(factors, b->factors, (size_t)b->nfac * 8, mem, b->stream))) return rc;
+    if (lambdas && (rc = copy_out(lambdas, b->lambdas, (size_t)b->nlam * 8, mem, b->stream))) return rc;
+    ITCPD_HIP(hipStreamSynchronize(b->stream));
+    return ITCPD_OK;
+}
+
+int itcpd_batch_set_threads(itcpd_batch* b, int32_t threads) {
+    if (!b) return fail(ITCPD_E_ARG, "null batch");
+    if (threads != 256 && threads != 512) return fail(ITCPD_E_ARG, "threads must be 256 or 512");
+    b->nt = threads;
+    return ITCPD_OK;
+}
+
+int itcpd_batch_run(itcpd_batch* b, int32_t sweeps, double* fits, double* elapsed_ms) {
+    if (!b) return fail(ITCPD_E_ARG, "null batch");
+    if (sweeps < 1) return fail(ITCPD_E_ARG, "sweeps must be >= 1");
+    if (!b->have_cpd) return fail(ITCPD_E_STATE, "itcpd_batch_run before itcpd_batch_set_cpd");
+    DeviceGuard dg(b->device);
+    int rc;
+    if ((rc = batch_launch_timed(b, sweeps, elapsed_ms))) return rc;
+    if (fits) ITCPD_HIP(hipMemcpy(fits, b->fits, (size_t)b->nprob * 8, hipMemcpyDeviceToHost));
+    return ITCPD_OK;
+}
+
+int itcpd_batch_tune(itcpd_batch* b, int32_t sweeps, int32_t* threads_chosen, double* ms_per_config) {
+    if (!b) return fail(ITCPD_E_ARG, "null batch");
+    if (sweeps < 1) return fail(ITCPD_E_ARG, "sweeps must be >= 1");
+    if (!b->have_cpd) return fail(ITCPD_E_STATE, "itcpd_batch_tune before itcpd_batch_set_cpd");
+    DeviceGuard dg(b->device);
+    ITCPD_HIP(hipMemcpyAsync(b->backup, b->factors, (size_t)b->nfac * 8, hipMemcpyDeviceToDevice, b->stream));
+    const int cand[2] = {256, 512};
+    double best = 1e300;
+    int best_nt = b->nt;
+    for (int c = 0; c < 2; ++c) {
+        b->nt = cand[c];
+        double ms = 0.0;
+        int rc;
+        if ((rc = batch_launch_timed(b, sweeps, &ms))) return rc;
+        if (ms_per_config) ms_per_config[c] = ms;
+        if (ms < best) { best = ms; best_nt = cand[c]; }
+        ITCPD_HIP(hipMemcpyAsync(b->factors, b->backup, (size_t)b->nfac * 8, hipMemcpyDeviceToDevice, b->stream));
+    }
+    ITCPD_HIP(hipStreamSynchronize(b->stream));
+    b->nt = best_nt;
+    if (threads_chosen) *threads_chosen = best_nt;
     return ITCPD_OK;
 }
 

@@ -1,0 +1,586 @@
+// Dense linear-algebra building blocks shared by the per-plan solve kernels and
+// the batched ALS kernel: Gram Hadamard products, block / wave reductions, the
+// DPP argmax, the column-pivoted QR minimum-norm solve (LAPACK xGEQP3 + xLAIC1 +
+// xTZRZF, restated) and the FitCheck / NoCheck state update.
+#pragma once
+#include "internal.h"
+
+namespace itcpd {
+
+struct GramSet {
+    const double* G[ITCPD_MAX_ORDER];
+    int N;
+};
+
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
+static constexpr int FACTOR_THREADS = 512;
+
+
+// Wave-wide argmax of (v, i): largest v, smallest i on ties; NaN never wins.
+// DPP row_ror 8/4/2/1 all-reduces each row of 16 lanes, then the four row
+// results are combined from readlane values (uniform result in every lane).
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp_step(double& v, int& i) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int olo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+    const int ohi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    const int oi = __builtin_amdgcn_update_dpp(0, i, CTRL, 0xF, 0xF, false);
+    const double ov = __hiloint2double(ohi, olo);
+    if (ov > v || (ov == v && oi < i)) {
+        v = ov;
+        i = oi;
+    }
+}
+__device__ __forceinline__ void wave_argmax(double& v, int& i) {
+    argmax_dpp_step<0x128>(v, i);   // row_ror:8
+    argmax_dpp_step<0x124>(v, i);   // row_ror:4
+    argmax_dpp_step<0x122>(v, i);   // row_ror:2
+    argmax_dpp_step<0x121>(v, i);   // row_ror:1
+    double bv = v;
+    int bi = i;
+    bv = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 0),
+                          __builtin_amdgcn_readlane(__double2loint(v), 0));
+    bi = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+    for (int r = 1; r < 4; ++r) {
+        const double rv = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 16 * r),
+                                           __builtin_amdgcn_readlane(__double2loint(v), 16 * r));
+        const int ri = __builtin_amdgcn_readlane(i, 16 * r);
+        if (rv > bv || (rv == bv && ri < bi)) {
+            bv = rv;
+            bi = ri;
+        }
+    }
+    v = bv;
+    i = bi;
+}
+
+// ----------------------------------------------------- batched global loads
+// Small kernels here are latency bound: every helper issues all of its loads
+// before the first use, so a thread waits on memory once per batch.
+
+// (1 .* G_a .* G_b ...) over the modes != skip, in mode order, at element x
+// (MttkrpAlgorithm.jl:38-47; with skip < 0 it is norm_factors' product).
+__device__ __forceinline__ double had_at(const GramSet& gs, int skip, int64_t x) {
+    double g[ITCPD_MAX_ORDER];
+#pragma unroll
+    for (int m = 0; m < ITCPD_MAX_ORDER; ++m)
+        g[m] = (m < gs.N && m != skip) ? gs.G[m][x] : 1.0;
+    double h = 1.0;
+#pragma unroll
+    for (int m = 0; m < ITCPD_MAX_ORDER; ++m) h *= g[m];
+    return h;
+}
+
+// sum_{b < n} p[b * stride + x] in increasing b, eight loads in flight.
+__device__ __forceinline__ double sum_parts_at(const double* __restrict__ p, int64_t stride, int n,
+                                               int64_t x) {
+    double acc = 0.0;
+    int b = 0;
+    for (; b + 8 <= n; b += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = p[(int64_t)(b + k) * stride + x];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k];
+    }
+    for (; b < n; ++b) acc += p[(int64_t)b * stride + x];
+    return acc;
+}
+
+// dst[x] = (1 .* G_a .* G_b ...)(x) over the modes != skip, in mode order, for
+// x < n: per mode, 16 element loads per thread are in flight together.
+template <int NT>
+__device__ __forceinline__ void fill_hadamard(double* dst, const GramSet& gs, int skip, int n) {
+    for (int x0 = threadIdx.x; x0 < n; x0 += NT * 16) {
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = 1.0;
+        for (int m = 0; m < gs.N; ++m) {
+            if (m == skip) continue;
+            const double* __restrict__ G = gs.G[m];
+            double g[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int x = x0 + NT * k;
+                g[k] = G[x < n ? x : 0];
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] *= g[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int x = x0 + NT * k;
+            if (x < n) dst[x] = v[k];
+        }
+    }
+}
+
+// --------------------------------------------------------- block reductions
+template <int NT>
+__device__ double block_sum(double v, double* red) {
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int o = NT / 2; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// First index of the maximum of f(i) over [lo, hi) (ties -> smaller index).
+// NaN entries never win (they compare false), matching a strict '>' scan.
+template <int NT, typename F>
+__device__ int block_argmax(int lo, int hi, F f, double* redv, int* redi) {
+    const int t = threadIdx.x;
+    double best = -__builtin_inf();
+    int bi = hi;
+    for (int i = lo + t; i < hi; i += NT) {
+        const double v = f(i);
+        if (v > best) {
+            best = v;
+            bi = i;
+        }
+    }
+    redv[t] = best;
+    redi[t] = bi;
+    __syncthreads();
+    for (int o = NT / 2; o > 0; o >>= 1) {
+        if (t < o) {
+            const double v2 = redv[t + o];
+            const int i2 = redi[t + o];
+            if (v2 > redv[t] || (v2 == redv[t] && i2 < redi[t])) {
+                redv[t] = v2;
+                redi[t] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    const int r = redi[0];
+    __syncthreads();
+    return r;
+}
+
+template <int RQ>
+__device__ __forceinline__ double bcast(const double (&y)[RQ], int j) {
+    const int q = j >> 6, ln = j & 63;
+    double v = 0.0;
+#pragma unroll
+    for (int qq = 0; qq < RQ; ++qq)
+        if (qq == q) v = y[qq];
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), ln);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), ln);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// QR path of the row solve (column-pivoted QR minimum-norm data in W):
+// y <- Q^T y, T11 solve on the leading rk entries, y <- Z^T y.
+template <int RQ>
+__device__ void fallback_apply(double (&y)[RQ], int lane, int R, int rk, const double* __restrict__ W,
+                               const double* __restrict__ qtau, const double* __restrict__ ztau) {
+    if (rk == 0) {
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) y[q] = 0.0;
+        return;
+    }
+    for (int tq = 0; tq < R; ++tq) {
+        const double tau = qtau[tq];
+        if (tau == 0.0) continue;
+        double part = 0.0;
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int k = lane + 64 * q;
+            if (k == tq) part += y[q];
+            else if (k > tq && k < R) part += W[k + (int64_t)R * tq] * y[q];
+        }
+        const double f = tau * wave_sum(part);
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int k = lane + 64 * q;
+            if (k == tq) y[q] -= f;
+            else if (k > tq && k < R) y[q] -= f * W[k + (int64_t)R * tq];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RQ; ++q)
+        if (lane + 64 * q >= rk) y[q] = 0.0;
+    for (int j = rk - 1; j >= 0; --j) {
+        const double wj = bcast<RQ>(y, j) / W[j + (int64_t)R * j];
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int k = lane + 64 * q;
+            if (k == j) y[q] = wj;
+            else if (k < j) y[q] -= W[k + (int64_t)R * j] * wj;
+        }
+    }
+    if (rk < R) {
+        for (int tq = 0; tq < rk; ++tq) {
+            const double tau = ztau[tq];
+            if (tau == 0.0) continue;
+            double part = 0.0;
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int k = lane + 64 * q;
+                if (k == tq) part += y[q];
+                else if (k >= rk && k < R) part += W[tq + (int64_t)R * k] * y[q];
+            }
+            const double f = tau * wave_sum(part);
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int k = lane + 64 * q;
+                if (k == tq) y[q] -= f;
+                else if (k >= rk && k < R) y[q] -= f * W[tq + (int64_t)R * k];
+            }
+        }
+    }
+}
+
+// The FitCheck / NoCheck update after a sweep (fit_check.jl:24-66, no_check.jl:8-20);
+// one thread.  fit = 1 - sqrt(|ref^2 + ||X||^2 - 2|<T,X>||) / ref.
+static __device__ void check_update(DevCheck* chk, double inner, double fact_square, double* scal,
+                             double* trace, int trace_cap) {
+    const double ref = chk->ref_norm;
+    const double normResidual = sqrt(fabs(ref * ref + fact_square - 2.0 * fabs(inner)));
+    const double fit = 1.0 - normResidual / ref;
+    scal[0] = inner;
+    scal[1] = fact_square;
+    scal[2] = fit;
+    if (chk->kind != ITCPD_CHECK_FIT) return;
+    chk->iter += 1;
+    const double dfit = fabs(chk->lastfit - fit);
+    chk->lastfit = fit;
+    if (trace && chk->sweeps < trace_cap) trace[chk->sweeps] = fit;
+    chk->sweeps += 1;
+    if (fit != fit) {
+        chk->nan_flag = 1;
+        chk->done = 1;
+        return;
+    }
+    if (dfit < chk->tolerance) {
+        chk->counter += 1;
+        if (chk->counter >= 2) {
+            chk->total_iter = chk->iter;
+            chk->iter = 0;
+            chk->counter = 0;
+            chk->final_fit = chk->lastfit;
+            chk->lastfit = 0.0;
+            chk->done = 1;
+            return;
+        }
+    } else {
+        chk->counter = 0;
+    }
+    if (chk->iter >= chk->max_counter) {
+        chk->total_iter = chk->iter;
+        chk->iter = 0;
+        chk->counter = 0;
+        chk->final_fit = chk->lastfit;
+        chk->lastfit = 0.0;
+    }
+}
+
+// ------------------------------------------------------ xLAIC1 restatement
+// One step of incremental condition estimation (Bischof): given the estimate
+// `sest` of the extreme singular value of an upper-triangular R_j with
+// approximate singular vector x, the new column w (alpha = x^T w) and the new
+// diagonal gamma, returns the estimate for R_{j+1} and the rotation (s, c) that
+// extends x.  job = 1: largest, job = 2: smallest singular value.
+static __device__ void laic1(int job, double alpha, double sest, double gamma, double* sestpr, double* s,
+                      double* c) {
+    const double eps = 1.1102230246251565e-16;  // LAPACK dlamch('Epsilon') = 2^-53
+    const double absalp = fabs(alpha), absgam = fabs(gamma), absest = fabs(sest);
+    if (job == 1) {
+        if (sest == 0.0) {
+            const double s1 = fmax(absgam, absalp);
+            if (s1 == 0.0) { *s = 0.0; *c = 1.0; *sestpr = 0.0; return; }
+            double ss = alpha / s1, cc = gamma / s1;
+            const double tmp = sqrt(ss * ss + cc * cc);
+            *s = ss / tmp; *c = cc / tmp; *sestpr = s1 * tmp;
+            return;
+        }
+        if (absgam <= eps * absest) {
+            *s = 1.0; *c = 0.0;
+            const double tmp = fmax(absest, absalp);
+            const double s1 = absest / tmp, s2 = absalp / tmp;
+            *sestpr = tmp * sqrt(s1 * s1 + s2 * s2);
+            return;
+        }
+        if (absalp <= eps * absest) {
+            if (absgam <= absest) { *s = 1.0; *c = 0.0; *sestpr = absest; }
+            else { *s = 0.0; *c = 1.0; *sestpr = absgam; }
+            return;
+        }
+        if (absest <= eps * absalp || absest <= eps * absgam) {
+            if (absgam <= absalp) {
+                const double tmp = absgam / absalp;
+                const double ss = sqrt(1.0 + tmp * tmp);
+                *sestpr = absalp * ss;
+                *c = (gamma / absalp) / ss;
+                *s = copysign(1.0, alpha) / ss;
+            } else {
+                const double tmp = absalp / absgam;
+                const double cc = sqrt(1.0 + tmp * tmp);
+                *sestpr = absgam * cc;
+                *s = (alpha / absgam) / cc;
+                *c = copysign(1.0, gamma) / cc;
+            }
+            return;
+        }
+        const double z1 = alpha / absest, z2 = gamma / absest;
+        const double b = (1.0 - z1 * z1 - z2 * z2) * 0.5;
+        const double cc = z1 * z1;
+        const double t = (b > 0.0) ? cc / (b + sqrt(b * b + cc)) : sqrt(b * b + cc) - b;
+        const double sine = -z1 / t, cosine = -z2 / (1.0 + t);
+        const double tmp = sqrt(sine * sine + cosine * cosine);
+        *s = sine / tmp; *c = cosine / tmp;
+        *sestpr = sqrt(t + 1.0) * absest;
+        return;
+    }
+    // job == 2: smallest singular value
+    if (sest == 0.0) {
+        *sestpr = 0.0;
+        double sine, cosine;
+        if (fmax(absgam, absalp) == 0.0) { sine = 1.0; cosine = 0.0; }
+        else { sine = -gamma; cosine = alpha; }
+        const double s1 = fmax(fabs(sine), fabs(cosine));
+        double ss = sine / s1, cc = cosine / s1;
+        const double tmp = sqrt(ss * ss + cc * cc);
+        *s = ss / tmp; *c = cc / tmp;
+        return;
+    }
+    if (absgam <= eps * absest) { *s = 0.0; *c = 1.0; *sestpr = absgam; return; }
+    if (absalp <= eps * absest) {
+        if (absgam <= absest) { *s = 0.0; *c = 1.0; *sestpr = absgam; }
+        else { *s = 1.0; *c = 0.0; *sestpr = absest; }
+        return;
+    }
+    if (absest <= eps * absalp || absest <= eps * absgam) {
+        if (absgam <= absalp) {
+            const double tmp = absgam / absalp;
+            const double cc = sqrt(1.0 + tmp * tmp);
+            *sestpr = absest * (tmp / cc);
+            *s = -(gamma / absalp) / cc;
+            *c = copysign(1.0, alpha) / cc;
+        } else {
+            const double tmp = absalp / absgam;
+            const double ss = sqrt(1.0 + tmp * tmp);
+            *sestpr = absest / ss;
+            *c = (alpha / absgam) / ss;
+            *s = -copysign(1.0, gamma) / ss;
+        }
+        return;
+    }
+    const double z1 = alpha / absest, z2 = gamma / absest;
+    const double norma = fmax(1.0 + z1 * z1 + fabs(z1 * z2), fabs(z1 * z2) + z2 * z2);
+    const double test = 1.0 + 2.0 * (z1 - z2) * (z1 + z2);
+    double sine, cosine;
+    if (test >= 0.0) {
+        const double b = (z1 * z1 + z2 * z2 + 1.0) * 0.5;
+        const double cc = z2 * z2;
+        const double t = cc / (b + sqrt(fabs(b * b - cc)));
+        sine = z1 / (1.0 - t);
+        cosine = -z2 / t;
+        *sestpr = sqrt(t + 4.0 * eps * eps * norma) * absest;
+    } else {
+        const double b = (z2 * z2 + z1 * z1 - 1.0) * 0.5;
+        const double cc = z1 * z1;
+        const double t = (b >= 0.0) ? -cc / (b + sqrt(b * b + cc)) : b - sqrt(b * b + cc);
+        sine = -z1 / t;
+        cosine = -z2 / (1.0 + t);
+        *sestpr = sqrt(1.0 + t + 4.0 * eps * eps * norma) * absest;
+    }
+    const double tmp = sqrt(sine * sine + cosine * cosine);
+    *s = sine / tmp; *c = cosine / tmp;
+}
+
+// Column-pivoted QR minimum-norm solve data for V (rank-deficient case), all NT
+// threads of the block: W = V (column-major) overwritten by the xGEQP3 reflectors,
+// rank by xLAIC1 incremental condition estimation at rcond = R*eps, then the
+// xTZRZF reduction of the trapezoid.  finfo = {chol_rank, 1, qr_rank}.
+template <int NT>
+__device__ void qr_fallback(const GramSet& gs, int mode, int R, double* __restrict__ W,
+                            double* __restrict__ qtau, double* __restrict__ ztau,
+                            int* __restrict__ jpvt, double* vwork, int* __restrict__ finfo,
+                            double* redv, int* redi, int rank) {
+    const int t = threadIdx.x;
+    if (t == 0) { finfo[0] = rank; finfo[1] = 1; }
+    // W = V (column-major; V is symmetric), recomputed from the Grams
+    fill_hadamard<NT>(W, gs, mode, R * R);
+    for (int i = t; i < R; i += NT) jpvt[i] = i;
+    __syncthreads();
+    double* vn1 = vwork;
+    double* vn2 = vwork + R;
+    // initial column norms
+    for (int j = t / 64; j < R; j += NT / 64) {
+        double ss = 0.0;
+        for (int k = (t & 63); k < R; k += 64) ss += W[k + R * j] * W[k + R * j];
+        for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+        if ((t & 63) == 0) { vn1[j] = sqrt(ss); vn2[j] = sqrt(ss); }
+    }
+    __syncthreads();
+    const double tol3z = 1.0536712127723509e-08;  // sqrt(dlamch('Epsilon')) = sqrt(2^-53)
+    for (int i = 0; i < R; ++i) {
+        const int p = block_argmax<NT>(i, R, [&](int j) { return fabs(vn1[j]); }, redv, redi);
+        const int pvt = (p < R) ? p : i;
+        if (pvt != i) {
+            for (int k = t; k < R; k += NT) {
+                const double a = W[k + R * pvt];
+                W[k + R * pvt] = W[k + R * i];
+                W[k + R * i] = a;
+            }
+            if (t == 0) {
+                const int q = jpvt[pvt]; jpvt[pvt] = jpvt[i]; jpvt[i] = q;
+                vn1[pvt] = vn1[i];
+                vn2[pvt] = vn2[i];
+            }
+        }
+        __syncthreads();
+        // Householder reflector for W(i:R, i) (xLARFG)
+        double xs = 0.0;
+        for (int k = i + 1 + t; k < R; k += NT) xs += W[k + R * i] * W[k + R * i];
+        const double xnorm2 = block_sum<NT>(xs, redv);
+        const double alpha = W[i + R * i];
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (xnorm2 > 0.0) {
+            const double xnorm = sqrt(xnorm2);
+            beta = -copysign(hypot(alpha, xnorm), alpha);
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        __syncthreads();
+        if (xnorm2 > 0.0)
+            for (int k = i + 1 + t; k < R; k += NT) W[k + R * i] *= scal;
+        __syncthreads();
+        if (t == 0) { W[i + R * i] = beta; qtau[i] = tau; }
+        __syncthreads();
+        // apply H = I - tau v v^T (v = [1; W(i+1:R, i)]) to columns i+1..R-1
+        if (tau != 0.0) {
+            for (int j = i + 1 + t / 64; j < R; j += NT / 64) {
+                double sdot = 0.0;
+                const int ln = t & 63;
+                if (ln == 0) sdot = W[i + R * j];
+                for (int k = i + 1 + ln; k < R; k += 64) sdot += W[k + R * i] * W[k + R * j];
+                for (int o = 32; o > 0; o >>= 1) sdot += __shfl_xor(sdot, o);
+                const double f = tau * sdot;
+                if (ln == 0) W[i + R * j] -= f;
+                for (int k = i + 1 + ln; k < R; k += 64) W[k + R * j] -= f * W[k + R * i];
+            }
+        }
+        __syncthreads();
+        // partial column-norm downdate (xLAQP2)
+        for (int j = i + 1 + t / 64; j < R; j += NT / 64) {
+            const int ln = t & 63;
+            const double v1 = vn1[j];
+            if (v1 != 0.0) {
+                double temp = fabs(W[i + R * j]) / v1;
+                temp = 1.0 - temp * temp;
+                temp = fmax(temp, 0.0);
+                const double q = v1 / vn2[j];
+                const double temp2 = temp * q * q;
+                if (temp2 <= tol3z) {
+                    double ss = 0.0;
+                    for (int k = i + 1 + ln; k < R; k += 64) ss += W[k + R * j] * W[k + R * j];
+                    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+                    if (ln == 0) { vn1[j] = sqrt(ss); vn2[j] = vn1[j]; }
+                } else if (ln == 0) {
+                    vn1[j] = v1 * sqrt(temp);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // numerical rank by incremental condition estimation, rcond = R * eps
+    __shared__ int s_rank;
+    if (t == 0) {
+        const double rcond = (double)R * 2.220446049250313e-16;
+        double* xmin = vwork;       // reuse: vn1/vn2 no longer needed
+        double* xmax = vwork + R;
+        double smax = fabs(W[0]);
+        double smin = smax;
+        int rk;
+        if (smax == 0.0) {
+            rk = 0;
+        } else {
+            rk = 1;
+            xmin[0] = 1.0;
+            xmax[0] = 1.0;
+            while (rk < R) {
+                const int i = rk;
+                double amin = 0.0, amax = 0.0;
+                for (int k = 0; k < i; ++k) {
+                    amin += xmin[k] * W[k + R * i];
+                    amax += xmax[k] * W[k + R * i];
+                }
+                const double gamma = W[i + R * i];
+                double sminpr, s1, c1, smaxpr, s2, c2;
+                laic1(2, amin, smin, gamma, &sminpr, &s1, &c1);
+                laic1(1, amax, smax, gamma, &smaxpr, &s2, &c2);
+                if (smaxpr * rcond <= sminpr) {
+                    for (int k = 0; k < i; ++k) { xmin[k] *= s1; xmax[k] *= s2; }
+                    xmin[i] = c1;
+                    xmax[i] = c2;
+                    smin = sminpr;
+                    smax = smaxpr;
+                    ++rk;
+                } else {
+                    break;
+                }
+            }
+        }
+        s_rank = rk;
+        finfo[2] = rk;
+    }
+    __syncthreads();
+    const int rk = s_rank;
+    if (rk == 0 || rk == R) return;
+
+    // RZ factorisation of the rk x R trapezoid [R11 R12] (xTZRZF / xLATRZ):
+    // for i = rk-1 .. 0, a reflector built from W(i,i) and W(i, rk:R) annihilates
+    // W(i, rk:R) and is applied from the right to rows 0..i-1.
+    const int L = R - rk;
+    for (int i = rk - 1; i >= 0; --i) {
+        double xs = 0.0;
+        for (int q = t; q < L; q += NT) xs += W[i + R * (rk + q)] * W[i + R * (rk + q)];
+        const double xnorm2 = block_sum<NT>(xs, redv);
+        const double alpha = W[i + R * i];
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (xnorm2 > 0.0) {
+            beta = -copysign(hypot(alpha, sqrt(xnorm2)), alpha);
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        __syncthreads();
+        if (xnorm2 > 0.0)
+            for (int q = t; q < L; q += NT) W[i + R * (rk + q)] *= scal;
+        __syncthreads();
+        if (t == 0) { W[i + R * i] = beta; ztau[i] = tau; }
+        __syncthreads();
+        if (tau != 0.0) {
+            // rows k < i: w = C(k,i) + sum_q C(k, rk+q) v_q ; C(k,i) -= tau w ; C(k,rk+q) -= tau w v_q
+            for (int k = t / 64; k < i; k += NT / 64) {
+                const int ln = t & 63;
+                double sdot = (ln == 0) ? W[k + R * i] : 0.0;
+                for (int q = ln; q < L; q += 64) sdot += W[k + R * (rk + q)] * W[i + R * (rk + q)];
+                for (int o = 32; o > 0; o >>= 1) sdot += __shfl_xor(sdot, o);
+                const double f = tau * sdot;
+                if (ln == 0) W[k + R * i] -= f;
+                for (int q = ln; q < L; q += 64) W[k + R * (rk + q)] -= f * W[i + R * (rk + q)];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+
+}  // namespace itcpd

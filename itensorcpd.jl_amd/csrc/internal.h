@@ -154,6 +154,33 @@ int launch_col_sumsq(itcpd_plan* p, int mode, const double* X);
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// ----------------------------------------------------------------- batches
+constexpr int BATCH_MAXD = 96;   // largest mode size of a batched problem
+constexpr int BATCH_MAXR = 32;   // largest rank of a batched problem
+
+struct BatchProblem {
+    int64_t toff;    // offset of the problem's tensor (elements, column-major)
+    int64_t foff;    // offset of its factors (A_0 | A_1 | A_2, each I_n x R column-major)
+    int I, J, K, R;
+    int loff;        // offset of its lambda
+    int index;       // position in the caller's list
+};
+
+struct BatchDev {
+    const BatchProblem* probs;
+    const int* order;        // queue order (largest problems first)
+    int nprob;
+    const double* T;
+    double* factors;
+    double* lambdas;
+    double* fits;
+    int* counter;
+    int grid;                // persistent workgroups
+};
+
+size_t batch_lds_bytes(int nt);
+int launch_batch(const BatchDev& b, int nt, int sweeps, hipStream_t st);
+
 }  // namespace itcpd
 
 #define ITCPD_HIP(call)                                              \

@@ -1,0 +1,157 @@
+"""Batched CP-ALS (one persistent kernel, graph-replayed) against the CPU oracle and
+the per-problem GPU plan.  Each problem runs NoCheck sweeps (no_check.jl:1-20); the
+batch applies the R x R solve as an explicit solution operator, so it agrees with
+the oracle's substitution solve to rounding (bar 1e-9 relative on factors and
+lambda for well-posed problems, rank <= every dimension)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def lowrank_fast(dims, R, noise, rng):
+    I, J, K = dims
+    A, B, Cf = (rng.standard_normal((d, R)) for d in dims)
+    T = ((Cf[:, None, :] * B[None, :, :]).reshape(J * K, R) @ A.T).reshape(K, J, I).transpose(2, 1, 0)
+    return np.asfortranarray(T + noise * rng.standard_normal(dims))
+
+
+def make_batch(n, lo, hi, rlo, rhi, seed, noise=1e-3, rank_le_dims=True):
+    rng = np.random.default_rng(seed)
+    Ts, Rs = [], []
+    for _ in range(n):
+        dims = tuple(int(x) for x in rng.integers(lo, hi + 1, 3))
+        R = int(rng.integers(rlo, rhi + 1))
+        if rank_le_dims:
+            R = min(R, min(dims))
+        Ts.append(lowrank_fast(dims, R, noise, rng))
+        Rs.append(R)
+    return Ts, Rs
+
+
+def check_against_oracle(got, refs, tol):
+    for b, (g, r) in enumerate(zip(got, refs)):
+        ferr = max(rel(x, y) for x, y in zip(g.factors, r.factors))
+        assert ferr < tol, (b, ferr)
+        assert rel(g.lam, r.lam) < tol, b
+
+
+def test_small_batch_graph_replays_match_oracle(itcpd):
+    Ts, Rs = make_batch(40, 8, 40, 1, 24, 1)
+    cur = [O.random_CPD(T, R, np.random.default_rng(100 + b)) for b, (T, R) in enumerate(zip(Ts, Rs))]
+    with itcpd.BatchALS(Ts, Rs) as bt:
+        bt.set_cpd([itcpd.CPD(c.factors, c.lam) for c in cur])
+        for rep in range(3):                       # three replays of the same graph
+            fits, ms = bt.run(3)
+            got = bt.get_cpd()
+            cur = [O.als_optimize(T, c, check=O.NoCheck(3)) for T, c in zip(Ts, cur)]
+            check_against_oracle(got, cur, 1e-9)
+            for T, g, f in zip(Ts, got, fits):
+                want = 1 - np.linalg.norm(O.reconstruct(g.factors, g.lam) - T) / np.linalg.norm(T)
+                assert abs(f - want) < 1e-9
+            assert ms > 0
+
+
+def test_batch_threads_and_tuner_restore_state(itcpd):
+    Ts, Rs = make_batch(24, 8, 64, 2, 32, 2)
+    init = [O.random_CPD(T, R, np.random.default_rng(200 + b)) for b, (T, R) in enumerate(zip(Ts, Rs))]
+    refs = [O.als_optimize(T, c, check=O.NoCheck(4)) for T, c in zip(Ts, init)]
+    for threads in (256, 512):
+        with itcpd.BatchALS(Ts, Rs) as bt:
+            bt.set_cpd([itcpd.CPD(c.factors, c.lam) for c in init])
+            bt.set_threads(threads)
+            bt.run(4)
+            check_against_oracle(bt.get_cpd(), refs, 1e-9)
+    with itcpd.BatchALS(Ts, Rs) as bt:
+        bt.set_cpd([itcpd.CPD(c.factors, c.lam) for c in init])
+        chosen, times = bt.tune(2)
+        assert chosen in (256, 512) and all(v > 0 for v in times.values())
+        bt.run(4)                                  # the tuner left the CPD as it was
+        check_against_oracle(bt.get_cpd(), refs, 1e-9)
+
+
+@pytest.mark.parametrize("threads", [256, 512])
+def test_largest_problems_match_oracle(itcpd, threads):
+    """Dimensions near 96 with ranks near 32: more (index, rank chunk) tasks than threads."""
+    Ts, Rs = make_batch(6, 80, 96, 25, 32, 5)
+    init = [O.random_CPD(T, R, np.random.default_rng(500 + b)) for b, (T, R) in enumerate(zip(Ts, Rs))]
+    refs = [O.als_optimize(T, c, check=O.NoCheck(2)) for T, c in zip(Ts, init)]
+    with itcpd.BatchALS(Ts, Rs) as bt:
+        bt.set_cpd([itcpd.CPD(c.factors, c.lam) for c in init])
+        bt.set_threads(threads)
+        bt.run(2)
+        check_against_oracle(bt.get_cpd(), refs, 1e-9)
+
+
+def test_batch_rank_deficient_uses_qr_fallback(itcpd):
+    rng = np.random.default_rng(3)
+    Ts = [np.asfortranarray(rng.standard_normal(d)) for d in [(12, 10, 8), (20, 9, 15), (16, 16, 16)]]
+    Rs = [5, 7, 12]
+    init = []
+    for b, (T, R) in enumerate(zip(Ts, Rs)):
+        c = O.random_CPD(T, R, np.random.default_rng(300 + b))
+        for f in c.factors:
+            f[:, 1] = f[:, 0]
+        init.append(c)
+    traces = [{} for _ in Ts]
+    refs = [O.optimize(T, c, check=O.NoCheck(3), trace=tr) for T, c, tr in zip(Ts, init, traces)]
+    assert all(tr["fallbacks"] == 9 for tr in traces)
+    with itcpd.BatchALS(Ts, Rs) as bt:
+        bt.set_cpd([itcpd.CPD(c.factors, c.lam) for c in init])
+        bt.run(3)
+        check_against_oracle(bt.get_cpd(), refs, 1e-8)
+
+
+def test_batch_errors(itcpd):
+    L = itcpd._lib
+    T = np.ones((8, 8, 8))
+    with pytest.raises(L.ITCPDError) as e:
+        itcpd.BatchALS([np.ones((97, 8, 8))], [4])
+    assert e.value.code == L.ITCPD_E_UNSUPPORTED
+    with pytest.raises(L.ITCPDError) as e:
+        itcpd.BatchALS([T], [33])
+    assert e.value.code == L.ITCPD_E_UNSUPPORTED
+    with itcpd.BatchALS([T], [2]) as bt:
+        with pytest.raises(L.ITCPDError) as e:
+            bt.run(2)
+        assert e.value.code == L.ITCPD_E_STATE
+        with pytest.raises(L.ITCPDError) as e:
+            bt.set_threads(128)
+        assert e.value.code == L.ITCPD_E_ARG
+
+
+def test_full_size_4096_problems_100_sweeps(itcpd):
+    """The batching workload at its real size: 4096 problems, I, J, K ~ U{8..96},
+    R ~ U{4..32}, N(0,1) factors + N(0, 1e-3), 100 sweeps in one launch.  Checks: every
+    fit is finite and the sampled problems agree with the per-problem GPU plan run
+    from the same init."""
+    rng = np.random.default_rng(4096)
+    Ts, Rs = [], []
+    for _ in range(4096):
+        dims = tuple(int(x) for x in rng.integers(8, 97, 3))
+        R = int(rng.integers(4, 33))
+        Ts.append(lowrank_fast(dims, R, 1e-3, rng))
+        Rs.append(R)
+    init = [O.random_CPD(T, R, np.random.default_rng(b)) for b, (T, R) in enumerate(zip(Ts, Rs))]
+    with itcpd.BatchALS(Ts, Rs) as bt:
+        bt.set_cpd([itcpd.CPD(c.factors, c.lam) for c in init])
+        bt.tune(1)
+        bt.set_cpd([itcpd.CPD(c.factors, c.lam) for c in init])
+        fits, ms = bt.run(100)
+        got = bt.get_cpd()
+    bad = np.flatnonzero(~np.isfinite(fits) | (fits > 1.0))
+    assert bad.size == 0, [(int(b), Ts[b].shape, Rs[b], float(fits[b])) for b in bad[:5]]
+    for b in (0, 1, 17, 1000, 2047, 4095):
+        with itcpd.ALSPlan(Ts[b], Rs[b]) as plan:
+            plan.set_cpd(itcpd.CPD(init[b].factors, init[b].lam))
+            chk = itcpd.FitCheck(0.0, 100, float(np.linalg.norm(Ts[b])))
+            plan.run(chk)
+            ref = plan.get_cpd()
+        assert max(rel(x, y) for x, y in zip(got[b].factors, ref.factors)) < 1e-7, b
+        assert abs(fits[b] - chk.final_fit) < 1e-9, b
