@@ -151,6 +151,11 @@ int itcpd_fit(itcpd_plan* plan, const double* M_last, int32_t mem, double ref_no
 /* norm(T) on the device (the ref_norm handed to FitCheck). */
 int itcpd_tensor_norm(itcpd_plan* plan, double* norm_out);
 
+/* Switches the MTTKRP arithmetic of an existing plan (no counterpart in the
+ * reference): the CPD, Grams and check state are kept, so a run can continue
+ * in another precision (e.g. BF16x3 sweeps, then FP32 or FP64 ones). */
+int itcpd_plan_set_precision(itcpd_plan* plan, int32_t prec);
+
 /* ---- instrumentation (no counterpart in the reference) ---- */
 int itcpd_plan_set_profiling(itcpd_plan* plan, int32_t on);
 int itcpd_get_kernel_stats(itcpd_plan* plan, int32_t kernel, itcpd_kernel_stats* stats);

@@ -59,6 +59,7 @@ SIGNATURES = {
     "itcpd_row_norm_post_solve": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
     "itcpd_fit": (C.c_int, [_P, _P, C.c_int32, C.c_double, _DP, _DP, _DP]),
     "itcpd_tensor_norm": (C.c_int, [_P, _DP]),
+    "itcpd_plan_set_precision": (C.c_int, [_P, C.c_int32]),
     "itcpd_plan_set_profiling": (C.c_int, [_P, C.c_int32]),
     "itcpd_get_kernel_stats": (C.c_int, [_P, C.c_int32, C.POINTER(KernelStats)]),
     "itcpd_synchronize": (C.c_int, [_P]),

@@ -329,6 +329,12 @@ class ALSPlan:
         L.check(self._lib.itcpd_tensor_norm(self._h, C.byref(out)))
         return out.value
 
+    def set_precision(self, prec):
+        """Continue this plan with another MTTKRP arithmetic ("fp64", "fp32", "bf16x3")."""
+        code = _prec_code(prec)
+        L.check(self._lib.itcpd_plan_set_precision(self._h, code))
+        self.prec = code
+
     def set_profiling(self, on: bool):
         L.check(self._lib.itcpd_plan_set_profiling(self._h, 1 if on else 0))
 

@@ -615,6 +615,20 @@ int itcpd_tensor_norm(itcpd_plan* p, double* norm_out) {
     return ITCPD_OK;
 }
 
+int itcpd_plan_set_precision(itcpd_plan* p, int32_t prec) {
+    if (!p) return fail(ITCPD_E_ARG, "null plan");
+    if (prec != ITCPD_FP64 && prec != ITCPD_FP32 && prec != ITCPD_BF16X3)
+        return fail(ITCPD_E_UNSUPPORTED, "prec must be ITCPD_FP64, ITCPD_FP32 or ITCPD_BF16X3");
+    DeviceGuard dg(p->device);
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    if (prec != ITCPD_FP64 && !p->Af) {
+        int rc;
+        if ((rc = dmalloc(&p->Af, (size_t)(p->maxI * p->R)))) return rc;
+    }
+    p->prec = prec;
+    return ITCPD_OK;
+}
+
 int itcpd_plan_set_profiling(itcpd_plan* p, int32_t on) {
     if (!p) return fail(ITCPD_E_ARG, "null plan");
     DeviceGuard dg(p->device);
