@@ -19,6 +19,8 @@ Other workloads (one JSON line each, same keys):
                            sweeps vs BF16x3 -> FP32 sweeps to reach the FP64 fit
                            within 1e-6 (fits evaluated in FP64 after the run)
     --workload batch       4096 small order-3 problems x 100 sweeps, one launch
+    --workload warmstart   train the warm-start network on 200^3 bump tensors, then
+                           time to fit from random_CPD vs the learned start
 """
 from __future__ import annotations
 
@@ -284,6 +286,74 @@ def bench_batch(args, E) -> dict:
     return res
 
 
+def bench_warmstart(args, E) -> dict:
+    from itensorcpd_jl_amd import warmstart as WS
+    D, K = 200, 48
+    t0 = time.perf_counter()
+    ws = WS.WarmStart((D, D, D), rmax=24, sketch_rank=K, hidden=(256, 256), seed=0)
+    X, Y = WS.training_pairs(ws, args.train_tensors, seed=1,
+                             log=lambda m: print(m, file=sys.stderr, flush=True))
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = WS.train(ws, X, Y, epochs=40, batch=256, lr=2e-3,
+                    log=lambda m: print(m, file=sys.stderr, flush=True))
+    t_train = time.perf_counter() - t0
+    rng = np.random.default_rng(12345)
+    rows = []
+    for t in range(args.eval_tensors):
+        R = int(rng.integers(8, 25))
+        fac, lam, sigma = WS.bump_problem((D, D, D), R, rng)
+        T = E.reconstruct(E.CPD(fac, lam), dtype=np.float32, noise_sigma=sigma, seed=777 + t)
+        nrm = float(np.linalg.norm(T.to_numpy().astype(np.float64)))
+
+        def traced(cp, max_sweeps, tol):
+            with E.ALSPlan(T, R) as plan:
+                plan.set_cpd(cp)
+                chk = E.FitCheck(tol, max_sweeps, nrm)
+                plan.synchronize()
+                a = time.perf_counter()
+                info = plan.run(chk)
+                b = time.perf_counter()
+            f = np.asarray(info["fits"])
+            return f, (b - a) / max(len(f), 1)
+
+        fr, ms_r = traced(E.random_CPD(T.shape, R, rng=t), 1000, 1e-10)
+        target = fr[-1] - 1e-6
+        hit = lambda f: (int(np.flatnonzero(f >= target)[0]) + 1) if np.any(f >= target) else None
+        a = time.perf_counter()
+        cp, chosen = ws.init(T, R, rng=t)
+        t_init = time.perf_counter() - a
+        fw, ms_w = traced(cp, 1000, 0.0)
+        s_r, s_w = hit(fr), hit(fw)
+        rows.append({"R": R, "fit_star": float(fr[-1]), "start": chosen,
+                     "sweeps_random": s_r, "sweeps_warm": s_w,
+                     "time_random_s": s_r * ms_r if s_r else None,
+                     "time_warm_s": (t_init + s_w * ms_w) if s_w else None,
+                     "init_s": t_init, "ms_per_sweep": 1e3 * ms_r})
+        ws.release()
+        T.free()
+        print(f"[bench] eval {t}: R={R} sweeps random {s_r} warm {s_w} ({chosen})", file=sys.stderr, flush=True)
+    ws.release()
+    tr = [r["time_random_s"] for r in rows if r["time_random_s"] and r["time_warm_s"]]
+    tw = [r["time_warm_s"] for r in rows if r["time_random_s"] and r["time_warm_s"]]
+    speed = float(np.median(tr) / np.median(tw)) if tr else None
+    return {
+        "metric": METRIC, "value": speed, "unit": "x (time-to-fit, random_CPD start / learned start)",
+        "n_gpus": 1, "steps": len(rows), "warmup": 0,
+        "ms_per_step": float(np.median([r["ms_per_sweep"] for r in rows])),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 (network f32)",
+        "data": "synthetic (200^3 FP32, Gaussian-bump factors, log-uniform lambda, 30 dB SNR)",
+        "config": {"workload": "learned warm start: 200^3 FP32, R ~ U{8..24}; sweeps / time until the "
+                               "FP64 fit is within 1e-6 of the converged random_CPD fit",
+                   "train_tensors": args.train_tensors, "train_pairs": int(X.shape[0]),
+                   "data_s": t_data, "train_s": t_train, "loss_first_last": [hist[0], hist[-1]],
+                   "median_sweeps_random": float(np.median([r["sweeps_random"] or 1001 for r in rows])),
+                   "median_sweeps_warm": float(np.median([r["sweeps_warm"] or 1001 for r in rows])),
+                   "tensors": rows},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def _blas_threads() -> int:
     try:
         from threadpoolctl import threadpool_info
@@ -300,7 +370,9 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--rank", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--workload", choices=["plain", "precision", "batch"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "precision", "batch", "warmstart"], default="plain")
+    ap.add_argument("--train-tensors", type=int, default=1000, help="warmstart: training tensors")
+    ap.add_argument("--eval-tensors", type=int, default=8, help="warmstart: held-out tensors")
     ap.add_argument("--max-sweeps", type=int, default=300, help="precision workload: sweep cap")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -314,7 +386,8 @@ def main() -> int:
     E = G.load_package()
     L = E._lib
     if args.workload != "plain":
-        out = bench_precision(args, E) if args.workload == "precision" else bench_batch(args, E)
+        out = {"precision": bench_precision, "batch": bench_batch,
+               "warmstart": bench_warmstart}[args.workload](args, E)
         print(json.dumps(out), flush=True)
         return 0
 

@@ -186,6 +186,48 @@ int itcpd_batch_set_threads(itcpd_batch* batch, int32_t threads);
  * keeps the fastest; ms_per_config (nullable) receives the 2 timings (256, 512). */
 int itcpd_batch_tune(itcpd_batch* batch, int32_t sweeps, int32_t* threads_chosen, double* ms_per_config);
 
+/* ---- device memory and synthetic tensors (no counterpart in the reference) ----
+ * For callers with no GPU runtime binding of their own (the ctypes / ccall hosts). */
+int itcpd_device_alloc(void** ptr, uint64_t bytes, int32_t device);
+int itcpd_device_free(void* ptr, int32_t device);
+/* kind: 0 host -> device, 1 device -> host, 2 device -> device */
+int itcpd_memcpy(void* dst, const void* src, uint64_t bytes, int32_t kind, int32_t device);
+
+/* Replaces reconstruct(cp) (reconstruct.jl:2-9) on the device: writes
+ * X = sum_r lambda_r a_1r o ... o a_Nr (column-major, FP64 or FP32) into the device
+ * buffer `out`, plus noise_sigma * N(0,1) noise from a Philox stream keyed by `seed`
+ * when noise_sigma > 0.  factors / lambda live in `mem`. */
+int itcpd_reconstruct(int32_t order, const int64_t* dims, int64_t rank, const double* const* factors,
+                      const double* lambda, int32_t mem, void* out, int32_t out_elt_bytes,
+                      double noise_sigma, uint64_t seed, int32_t device);
+
+/* ---- warm-start network (no counterpart: the reference starts from random_CPD,
+ * cpd.jl:63-82) ----
+ * A fully connected network, widths[0] -> ... -> widths[nlayers], ReLU between
+ * layers, linear output, FP32; trained with Adam on 0.5 * sum(mask (Y - Y*)^2) / batch.
+ * Row-major batches (batch x width).  X / Y / targets / mask live in `mem`. */
+typedef struct itcpd_mlp itcpd_mlp;
+int itcpd_mlp_create(itcpd_mlp** mlp, int32_t nlayers, const int32_t* widths, int32_t batch_cap,
+                     uint64_t seed, int32_t device);
+int itcpd_mlp_destroy(itcpd_mlp* mlp);
+int itcpd_mlp_num_params(itcpd_mlp* mlp, int64_t* n);
+/* parameters: W_0 | b_0 | W_1 | b_1 | ...; W_l is widths[l] x widths[l+1] row-major */
+int itcpd_mlp_get_params(itcpd_mlp* mlp, float* params, int32_t mem);
+int itcpd_mlp_set_params(itcpd_mlp* mlp, const float* params, int32_t mem);
+int itcpd_mlp_forward(itcpd_mlp* mlp, int32_t batch, const float* X, int32_t mem, float* Y);
+/* Candidate factor columns from mode sketches (the warm start's network input):
+ * for each of nsk sketches S_s (D x K column-major, concatenated), ncand rounds of
+ * successive projection (largest-norm row -> direction d, candidate S d, deflate
+ * S -= (S d) d^T); out[(s * ncand + k) * D + i] = candidate k of sketch s at row i,
+ * scaled to unit maximum magnitude.  D <= 256, ncand <= K <= 64. */
+int itcpd_sketch_candidates(int32_t nsk, int32_t D, int32_t K, int32_t ncand, const double* S,
+                            int32_t mem, float* out, int32_t device);
+/* gradient of the loss (same layout as the parameters) without an update */
+int itcpd_mlp_gradients(itcpd_mlp* mlp, int32_t batch, const float* X, const float* Y_target,
+                        const float* mask, int32_t mem, float* grads, double* loss);
+int itcpd_mlp_train_step(itcpd_mlp* mlp, int32_t batch, const float* X, const float* Y_target,
+                         const float* mask, int32_t mem, float lr, double* loss);
+
 /* Thread-local text of the last failing call on this thread ("" if none). */
 const char* itcpd_last_error(void);
 

@@ -14,14 +14,14 @@ a test written like the reference's ``test/cp_als.jl``) reads the same:
 All arithmetic runs in libitcpd_hip.so on the GPU; there is no CPU fallback.
 """
 from .api import (  # noqa: F401
-    CPD, ALSPlan, BatchALS, CPDFit, FitCheck, KRPFreeNormal, KRPNormal, NoCheck, CPDOptimizer,
+    CPD, ALSPlan, BatchALS, DeviceTensor, MLP, reconstruct, CPDFit, FitCheck, KRPFreeNormal, KRPNormal, NoCheck, CPDOptimizer,
     als_optimize, cp_rank, decompose, decompose_adaptive, decompose_batch, increase_cpd_rank,
     random_CPD, tensor_norm,
 )
 from ._lib import ITCPDError  # noqa: F401
 
 __all__ = [
-    "CPD", "ALSPlan", "BatchALS", "decompose_batch", "CPDFit", "FitCheck", "KRPFreeNormal", "KRPNormal", "NoCheck",
+    "CPD", "ALSPlan", "BatchALS", "decompose_batch", "DeviceTensor", "MLP", "reconstruct", "CPDFit", "FitCheck", "KRPFreeNormal", "KRPNormal", "NoCheck",
     "CPDOptimizer", "als_optimize", "cp_rank", "decompose", "decompose_adaptive",
     "increase_cpd_rank", "random_CPD", "tensor_norm", "ITCPDError",
 ]

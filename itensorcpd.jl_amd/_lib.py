@@ -71,6 +71,22 @@ SIGNATURES = {
     "itcpd_batch_run": (C.c_int, [_P, C.c_int32, _DP, _DP]),
     "itcpd_batch_set_threads": (C.c_int, [_P, C.c_int32]),
     "itcpd_batch_tune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _DP]),
+    "itcpd_device_alloc": (C.c_int, [C.POINTER(_P), C.c_uint64, C.c_int32]),
+    "itcpd_device_free": (C.c_int, [_P, C.c_int32]),
+    "itcpd_memcpy": (C.c_int, [_P, _P, C.c_uint64, C.c_int32, C.c_int32]),
+    "itcpd_reconstruct": (C.c_int, [C.c_int32, _I64P, C.c_int64, C.POINTER(_P), _P, C.c_int32, _P,
+                                    C.c_int32, C.c_double, C.c_uint64, C.c_int32]),
+    "itcpd_mlp_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(C.c_int32), C.c_int32,
+                                   C.c_uint64, C.c_int32]),
+    "itcpd_mlp_destroy": (C.c_int, [_P]),
+    "itcpd_mlp_num_params": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "itcpd_mlp_get_params": (C.c_int, [_P, _P, C.c_int32]),
+    "itcpd_mlp_set_params": (C.c_int, [_P, _P, C.c_int32]),
+    "itcpd_mlp_forward": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P]),
+    "itcpd_mlp_gradients": (C.c_int, [_P, C.c_int32, _P, _P, _P, C.c_int32, _P, _DP]),
+    "itcpd_mlp_train_step": (C.c_int, [_P, C.c_int32, _P, _P, _P, C.c_int32, C.c_float, _DP]),
+    "itcpd_sketch_candidates": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, C.c_int32, _P,
+                                          C.c_int32]),
     "itcpd_last_error": (C.c_char_p, []),
     "itcpd_version": (C.c_char_p, []),
 }

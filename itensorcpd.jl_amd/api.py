@@ -122,10 +122,79 @@ def _colmajor_torch(t):
     return t.permute(*rev).contiguous().permute(*rev)
 
 
+class DeviceTensor:
+    """A column-major tensor in device memory owned by the engine (for hosts with no
+    GPU array library).  Plans bind it without a copy."""
+
+    def __init__(self, dims, dtype=np.float64, device: int = 0):
+        self.dims = tuple(int(d) for d in dims)
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.float64, np.float32):
+            raise ValueError("DeviceTensor holds float64 or float32")
+        self.device = int(device)
+        self.nbytes = int(np.prod(self.dims)) * self.dtype.itemsize
+        lib = L.load()
+        ptr = C.c_void_p()
+        L.check(lib.itcpd_device_alloc(C.byref(ptr), self.nbytes, self.device))
+        self.ptr = ptr.value
+        self._lib = lib
+
+    @classmethod
+    def from_numpy(cls, a, device: int = 0):
+        a = np.asfortranarray(a)
+        t = cls(a.shape, a.dtype, device)
+        L.check(t._lib.itcpd_memcpy(C.c_void_p(t.ptr), C.c_void_p(a.ctypes.data), t.nbytes, 0, t.device))
+        return t
+
+    def to_numpy(self):
+        a = np.empty(self.dims, dtype=self.dtype, order="F")
+        L.check(self._lib.itcpd_memcpy(C.c_void_p(a.ctypes.data), C.c_void_p(self.ptr), self.nbytes, 1,
+                                       self.device))
+        return a
+
+    @property
+    def shape(self):
+        return self.dims
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self._lib.itcpd_device_free(C.c_void_p(self.ptr), self.device)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def reconstruct(cp: "CPD", out: Optional[DeviceTensor] = None, dtype=np.float64, noise_sigma: float = 0.0,
+                seed: int = 0) -> DeviceTensor:
+    """reconstruct.jl:2-9 on the device: sum_r lambda_r a_1r o ... o a_Nr, into `out` (or a
+    new DeviceTensor), plus noise_sigma * N(0,1) from a Philox stream keyed by `seed`."""
+    dims = tuple(f.shape[0] for f in cp.factors)
+    if out is None:
+        out = DeviceTensor(dims, dtype)
+    elif out.dims != dims:
+        raise ValueError("output dims do not match the CPD")
+    fs = [np.asfortranarray(f, dtype=np.float64) for f in cp.factors]
+    lam = np.ascontiguousarray(cp.lam, dtype=np.float64)
+    cd = (C.c_int64 * len(dims))(*dims)
+    L.check(out._lib.itcpd_reconstruct(len(dims), cd, cp.rank, _ptr_array([f.ctypes.data for f in fs]),
+                                       C.c_void_p(lam.ctypes.data), L.ITCPD_MEM_HOST, C.c_void_p(out.ptr),
+                                       out.dtype.itemsize, float(noise_sigma), int(seed) & (2**64 - 1),
+                                       out.device))
+    return out
+
+
 class _Target:
     """Resolves a tensor argument to (dims, elt_bytes, pointer, mem, keep-alive)."""
 
     def __init__(self, A):
+        if isinstance(A, DeviceTensor):
+            self.dims, self.elt, self.ptr = A.dims, A.dtype.itemsize, A.ptr
+            self.mem, self.device, self.keep = L.ITCPD_MEM_DEVICE, A.device, A
+            return
         if _is_torch(A):
             import torch
             if A.is_cuda:
@@ -566,3 +635,80 @@ def decompose_batch(tensors, ranks, sweeps: int = 100, rng=None, tune: bool = Tr
             bt.tune(2)
         fits, _ = bt.run(sweeps)
         return bt.get_cpd(), fits
+
+
+# ------------------------------------------------------- warm-start network
+class MLP:
+    """The warm-start network on the device (ReLU MLP, FP32, Adam); see include/itcpd.h."""
+
+    def __init__(self, widths, batch_cap: int, seed: int = 0, device: int = 0):
+        lib = L.load()
+        self.widths = [int(w) for w in widths]
+        self.batch_cap = int(batch_cap)
+        w = (C.c_int32 * len(self.widths))(*self.widths)
+        h = C.c_void_p()
+        L.check(lib.itcpd_mlp_create(C.byref(h), len(self.widths) - 1, w, self.batch_cap, int(seed), int(device)))
+        self._h, self._lib = h, lib
+        n = C.c_int64()
+        L.check(lib.itcpd_mlp_num_params(h, C.byref(n)))
+        self.nparams = n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.itcpd_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def _f32(a):
+        return np.ascontiguousarray(a, dtype=np.float32)
+
+    def params(self):
+        out = np.empty(self.nparams, dtype=np.float32)
+        L.check(self._lib.itcpd_mlp_get_params(self._h, C.c_void_p(out.ctypes.data), L.ITCPD_MEM_HOST))
+        return out
+
+    def set_params(self, p):
+        p = self._f32(p)
+        if p.size != self.nparams:
+            raise ValueError("parameter count mismatch")
+        L.check(self._lib.itcpd_mlp_set_params(self._h, C.c_void_p(p.ctypes.data), L.ITCPD_MEM_HOST))
+
+    def forward(self, X):
+        X = self._f32(X)
+        Y = np.empty((X.shape[0], self.widths[-1]), dtype=np.float32)
+        L.check(self._lib.itcpd_mlp_forward(self._h, X.shape[0], C.c_void_p(X.ctypes.data), L.ITCPD_MEM_HOST,
+                                            C.c_void_p(Y.ctypes.data)))
+        return Y
+
+    def gradients(self, X, Yt, mask=None):
+        X, Yt = self._f32(X), self._f32(Yt)
+        mk = None if mask is None else self._f32(mask)
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = C.c_double()
+        L.check(self._lib.itcpd_mlp_gradients(self._h, X.shape[0], C.c_void_p(X.ctypes.data),
+                                              C.c_void_p(Yt.ctypes.data),
+                                              C.c_void_p(mk.ctypes.data) if mk is not None else None,
+                                              L.ITCPD_MEM_HOST, C.c_void_p(g.ctypes.data), C.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, X, Yt, mask=None, lr: float = 1e-3):
+        X, Yt = self._f32(X), self._f32(Yt)
+        mk = None if mask is None else self._f32(mask)
+        loss = C.c_double()
+        L.check(self._lib.itcpd_mlp_train_step(self._h, X.shape[0], C.c_void_p(X.ctypes.data),
+                                               C.c_void_p(Yt.ctypes.data),
+                                               C.c_void_p(mk.ctypes.data) if mk is not None else None,
+                                               L.ITCPD_MEM_HOST, float(lr), C.byref(loss)))
+        return loss.value

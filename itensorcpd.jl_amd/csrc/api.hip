@@ -23,30 +23,6 @@ int hip_fail(hipError_t e, const char* what) {
 
 namespace {
 
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
-template <typename T>
-int dmalloc(T** ptr, size_t count) {
-    *ptr = nullptr;
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void**)ptr, count * sizeof(T));
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
-    return ITCPD_OK;
-}
-
-void dfree(void* p) {
-    if (p) (void)hipFree(p);
-}
-
 int choose_rb(int64_t R) {
     if (R <= 16) return 1;
     if (R <= 32) return 2;
@@ -69,19 +45,6 @@ int choose_split(int N, const int64_t* dims) {
         }
     }
     return best;
-}
-
-int copy_in(void* dst, const void* src, size_t bytes, int mem, hipStream_t st) {
-    ITCPD_HIP(hipMemcpyAsync(dst, src, bytes, mem == ITCPD_MEM_DEVICE ? hipMemcpyDeviceToDevice
-                                                                      : hipMemcpyHostToDevice,
-                             st));
-    return ITCPD_OK;
-}
-int copy_out(void* dst, const void* src, size_t bytes, int mem, hipStream_t st) {
-    ITCPD_HIP(hipMemcpyAsync(dst, src, bytes, mem == ITCPD_MEM_DEVICE ? hipMemcpyDeviceToDevice
-                                                                      : hipMemcpyDeviceToHost,
-                             st));
-    return ITCPD_OK;
 }
 
 // Copies an I x R column-major block into an I x Rp column-major device array.
@@ -889,6 +852,307 @@ int itcpd_batch_tune(itcpd_batch* b, int32_t sweeps, int32_t* threads_chosen, do
     ITCPD_HIP(hipStreamSynchronize(b->stream));
     b->nt = best_nt;
     if (threads_chosen) *threads_chosen = best_nt;
+    return ITCPD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ device utilities
+extern "C" {
+
+int itcpd_device_alloc(void** ptr, uint64_t bytes, int32_t device) {
+    if (!ptr) return fail(ITCPD_E_ARG, "null pointer argument");
+    *ptr = nullptr;
+    DeviceGuard dg(device);
+    const hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    return ITCPD_OK;
+}
+
+int itcpd_device_free(void* ptr, int32_t device) {
+    DeviceGuard dg(device);
+    if (ptr) ITCPD_HIP(hipFree(ptr));
+    return ITCPD_OK;
+}
+
+int itcpd_memcpy(void* dst, const void* src, uint64_t bytes, int32_t kind, int32_t device) {
+    if ((!dst || !src) && bytes) return fail(ITCPD_E_ARG, "null pointer argument");
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    if (kind < 0 || kind > 2) return fail(ITCPD_E_ARG, "kind must be 0 (to device), 1 (to host) or 2");
+    DeviceGuard dg(device);
+    ITCPD_HIP(hipMemcpy(dst, src, bytes, k));
+    return ITCPD_OK;
+}
+
+int itcpd_reconstruct(int32_t order, const int64_t* dims, int64_t rank, const double* const* factors,
+                      const double* lambda, int32_t mem, void* out, int32_t out_elt_bytes,
+                      double noise_sigma, uint64_t seed, int32_t device) {
+    if (order < 1 || order > ITCPD_MAX_ORDER) return fail(ITCPD_E_ARG, "order must be between 1 and 8");
+    if (!dims || !factors || !lambda || !out) return fail(ITCPD_E_ARG, "null argument");
+    if (rank < 1) return fail(ITCPD_E_ARG, "rank must be >= 1");
+    if (out_elt_bytes != 8 && out_elt_bytes != 4) return fail(ITCPD_E_ARG, "out_elt_bytes must be 8 or 4");
+    for (int n = 0; n < order; ++n)
+        if (dims[n] < 1) return fail(ITCPD_E_ARG, "every dimension must be >= 1");
+    DeviceGuard dg(device);
+    hipStream_t st;
+    ITCPD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int64_t nfac = 0;
+    for (int n = 0; n < order; ++n) nfac += dims[n] * rank;
+    double* buf = nullptr;
+    int rc = dmalloc(&buf, (size_t)(nfac + rank));
+    const double* fdev[ITCPD_MAX_ORDER];
+    int64_t off = 0;
+    for (int n = 0; n < order && !rc; ++n) {
+        rc = copy_in(buf + off, factors[n], (size_t)(dims[n] * rank) * 8, mem, st);
+        fdev[n] = buf + off;
+        off += dims[n] * rank;
+    }
+    if (!rc) rc = copy_in(buf + off, lambda, (size_t)rank * 8, mem, st);
+    if (!rc) rc = launch_reconstruct(order, dims, rank, fdev, buf + off, out, out_elt_bytes, noise_sigma, seed, st);
+    const hipError_t e = hipStreamSynchronize(st);
+    dfree(buf);
+    (void)hipStreamDestroy(st);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    return ITCPD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- the MLP
+struct itcpd_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    itcpd::MlpDev m;
+    float* xs = nullptr;   // host-input staging: batch_cap x in
+    float* ys = nullptr;   // batch_cap x out (targets)
+    float* ms = nullptr;   // batch_cap x out (mask)
+};
+
+namespace itcpd {
+namespace {
+void mlp_free(itcpd_mlp* q) {
+    if (!q) return;
+    DeviceGuard dg(q->device);
+    if (q->stream) (void)hipStreamSynchronize(q->stream);
+    MlpDev& m = q->m;
+    dfree(m.params); dfree(m.grads); dfree(m.adam_m); dfree(m.adam_v);
+    for (int l = 0; l < MLP_MAX_LAYERS; ++l) dfree(m.act[l]);
+    dfree(m.dA); dfree(m.dB); dfree(m.loss_part); dfree(m.loss);
+    dfree(q->xs); dfree(q->ys); dfree(q->ms);
+    if (q->stream) (void)hipStreamDestroy(q->stream);
+    delete q;
+}
+
+// stage host inputs (mem == HOST) into the network's buffers
+int mlp_stage(itcpd_mlp* q, int batch, const float* X, const float* Yt, const float* mask, int mem,
+              const float** Xd, const float** Yd, const float** Md) {
+    const MlpDev& m = q->m;
+    const size_t nin = (size_t)batch * m.widths[0], nout = (size_t)batch * m.widths[m.nlayers];
+    if (mem == ITCPD_MEM_DEVICE) {
+        *Xd = X; if (Yd) *Yd = Yt; if (Md) *Md = mask;
+        return ITCPD_OK;
+    }
+    int rc;
+    if ((rc = copy_in(q->xs, X, nin * 4, ITCPD_MEM_HOST, q->stream))) return rc;
+    *Xd = q->xs;
+    if (Yd) {
+        if ((rc = copy_in(q->ys, Yt, nout * 4, ITCPD_MEM_HOST, q->stream))) return rc;
+        *Yd = q->ys;
+    }
+    if (Md) {
+        if (mask && (rc = copy_in(q->ms, mask, nout * 4, ITCPD_MEM_HOST, q->stream))) return rc;
+        *Md = mask ? q->ms : nullptr;
+    }
+    return ITCPD_OK;
+}
+
+int mlp_check(itcpd_mlp* q, int batch, const void* X) {
+    if (!q || !X) return fail(ITCPD_E_ARG, "null argument");
+    if (batch < 1 || batch > q->m.batch_cap) return fail(ITCPD_E_ARG, "batch must be in [1, batch_cap]");
+    return ITCPD_OK;
+}
+}  // namespace
+}  // namespace itcpd
+
+extern "C" {
+
+int itcpd_mlp_create(itcpd_mlp** out, int32_t nlayers, const int32_t* widths, int32_t batch_cap,
+                     uint64_t seed, int32_t device) {
+    if (!out || !widths) return fail(ITCPD_E_ARG, "null argument");
+    *out = nullptr;
+    if (nlayers < 1 || nlayers > MLP_MAX_LAYERS) return fail(ITCPD_E_ARG, "nlayers must be in [1, 6]");
+    if (batch_cap < 1) return fail(ITCPD_E_ARG, "batch_cap must be >= 1");
+    for (int l = 0; l <= nlayers; ++l)
+        if (widths[l] < 1) return fail(ITCPD_E_ARG, "every width must be >= 1");
+    itcpd_mlp* q = new (std::nothrow) itcpd_mlp();
+    if (!q) return fail(ITCPD_E_OOM, "host allocation failed");
+    DeviceGuard dg(device);
+    q->device = device;
+    MlpDev& m = q->m;
+    m.nlayers = nlayers;
+    m.batch_cap = batch_cap;
+    int maxw = 0;
+    for (int l = 0; l <= nlayers; ++l) { m.widths[l] = widths[l]; maxw = std::max(maxw, widths[l]); }
+    int64_t np = 0;
+    for (int l = 0; l < nlayers; ++l) np += (int64_t)widths[l] * widths[l + 1] + widths[l + 1];
+    m.nparams = np;
+    auto bail = [&](int code) { mlp_free(q); return code; };
+    hipError_t e = hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+    int rc;
+    if ((rc = dmalloc(&m.params, (size_t)np))) return bail(rc);
+    if ((rc = dmalloc(&m.grads, (size_t)np))) return bail(rc);
+    if ((rc = dmalloc(&m.adam_m, (size_t)np))) return bail(rc);
+    if ((rc = dmalloc(&m.adam_v, (size_t)np))) return bail(rc);
+    for (int l = 0; l < nlayers; ++l)
+        if ((rc = dmalloc(&m.act[l], (size_t)batch_cap * widths[l + 1]))) return bail(rc);
+    if ((rc = dmalloc(&m.dA, (size_t)batch_cap * maxw))) return bail(rc);
+    if ((rc = dmalloc(&m.dB, (size_t)batch_cap * maxw))) return bail(rc);
+    if ((rc = dmalloc(&m.loss_part, (size_t)((int64_t)batch_cap * widths[nlayers] / 256 + 2)))) return bail(rc);
+    if ((rc = dmalloc(&m.loss, 1))) return bail(rc);
+    if ((rc = dmalloc(&q->xs, (size_t)batch_cap * widths[0]))) return bail(rc);
+    if ((rc = dmalloc(&q->ys, (size_t)batch_cap * widths[nlayers]))) return bail(rc);
+    if ((rc = dmalloc(&q->ms, (size_t)batch_cap * widths[nlayers]))) return bail(rc);
+    // He-uniform weights (bound sqrt(6 / fan_in)), zero biases; host stream seeded by `seed`
+    std::vector<float> h((size_t)np);
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    auto next = [&]() {   // splitmix64
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    int64_t off = 0;
+    for (int l = 0; l < nlayers; ++l) {
+        const int64_t nw = (int64_t)widths[l] * widths[l + 1];
+        const float bound = std::sqrt(6.0f / (float)widths[l]);
+        for (int64_t i = 0; i < nw; ++i) {
+            const double u = (double)(next() >> 11) * (1.0 / 9007199254740992.0);
+            h[off + i] = (float)((2.0 * u - 1.0) * bound);
+        }
+        m.W[l] = m.params + off;
+        m.gW[l] = m.grads + off;
+        off += nw;
+        for (int j = 0; j < widths[l + 1]; ++j) h[off + j] = 0.f;
+        m.b[l] = m.params + off;
+        m.gb[l] = m.grads + off;
+        off += widths[l + 1];
+    }
+    e = hipMemcpy(m.params, h.data(), (size_t)np * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(m.adam_m, 0, (size_t)np * 4);
+    if (e == hipSuccess) e = hipMemset(m.adam_v, 0, (size_t)np * 4);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipMemcpy(mlp init)"));
+    *out = q;
+    return ITCPD_OK;
+}
+
+int itcpd_mlp_destroy(itcpd_mlp* q) {
+    mlp_free(q);
+    return ITCPD_OK;
+}
+
+int itcpd_mlp_num_params(itcpd_mlp* q, int64_t* n) {
+    if (!q || !n) return fail(ITCPD_E_ARG, "null argument");
+    *n = q->m.nparams;
+    return ITCPD_OK;
+}
+
+int itcpd_mlp_get_params(itcpd_mlp* q, float* out, int32_t mem) {
+    if (!q || !out) return fail(ITCPD_E_ARG, "null argument");
+    DeviceGuard dg(q->device);
+    int rc;
+    if ((rc = copy_out(out, q->m.params, (size_t)q->m.nparams * 4, mem, q->stream))) return rc;
+    ITCPD_HIP(hipStreamSynchronize(q->stream));
+    return ITCPD_OK;
+}
+
+int itcpd_mlp_set_params(itcpd_mlp* q, const float* in, int32_t mem) {
+    if (!q || !in) return fail(ITCPD_E_ARG, "null argument");
+    DeviceGuard dg(q->device);
+    int rc;
+    if ((rc = copy_in(q->m.params, in, (size_t)q->m.nparams * 4, mem, q->stream))) return rc;
+    ITCPD_HIP(hipStreamSynchronize(q->stream));
+    return ITCPD_OK;
+}
+
+int itcpd_mlp_forward(itcpd_mlp* q, int32_t batch, const float* X, int32_t mem, float* Y) {
+    int rc;
+    if ((rc = mlp_check(q, batch, X))) return rc;
+    if (!Y) return fail(ITCPD_E_ARG, "null output");
+    DeviceGuard dg(q->device);
+    const float* Xd;
+    if ((rc = mlp_stage(q, batch, X, nullptr, nullptr, mem, &Xd, nullptr, nullptr))) return rc;
+    if ((rc = mlp_forward(q->m, batch, Xd, q->stream))) return rc;
+    const MlpDev& m = q->m;
+    if ((rc = copy_out(Y, m.act[m.nlayers - 1], (size_t)batch * m.widths[m.nlayers] * 4, mem, q->stream))) return rc;
+    ITCPD_HIP(hipStreamSynchronize(q->stream));
+    return ITCPD_OK;
+}
+
+int itcpd_mlp_gradients(itcpd_mlp* q, int32_t batch, const float* X, const float* Yt, const float* mask,
+                        int32_t mem, float* grads, double* loss) {
+    int rc;
+    if ((rc = mlp_check(q, batch, X))) return rc;
+    if (!Yt) return fail(ITCPD_E_ARG, "null target");
+    DeviceGuard dg(q->device);
+    const float *Xd, *Yd, *Md;
+    if ((rc = mlp_stage(q, batch, X, Yt, mask, mem, &Xd, &Yd, &Md))) return rc;
+    if ((rc = mlp_forward(q->m, batch, Xd, q->stream))) return rc;
+    if ((rc = mlp_backward(q->m, batch, Xd, Yd, Md, q->m.loss, q->stream))) return rc;
+    if (grads && (rc = copy_out(grads, q->m.grads, (size_t)q->m.nparams * 4, mem, q->stream))) return rc;
+    if (loss) ITCPD_HIP(hipMemcpyAsync(loss, q->m.loss, 8, hipMemcpyDeviceToHost, q->stream));
+    ITCPD_HIP(hipStreamSynchronize(q->stream));
+    return ITCPD_OK;
+}
+
+int itcpd_mlp_train_step(itcpd_mlp* q, int32_t batch, const float* X, const float* Yt, const float* mask,
+                         int32_t mem, float lr, double* loss) {
+    int rc;
+    if ((rc = mlp_check(q, batch, X))) return rc;
+    if (!Yt) return fail(ITCPD_E_ARG, "null target");
+    DeviceGuard dg(q->device);
+    const float *Xd, *Yd, *Md;
+    if ((rc = mlp_stage(q, batch, X, Yt, mask, mem, &Xd, &Yd, &Md))) return rc;
+    if ((rc = mlp_forward(q->m, batch, Xd, q->stream))) return rc;
+    if ((rc = mlp_backward(q->m, batch, Xd, Yd, Md, q->m.loss, q->stream))) return rc;
+    if ((rc = mlp_adam(q->m, lr, q->stream))) return rc;
+    if (loss) ITCPD_HIP(hipMemcpyAsync(loss, q->m.loss, 8, hipMemcpyDeviceToHost, q->stream));
+    ITCPD_HIP(hipStreamSynchronize(q->stream));   // host inputs may be reused by the caller
+    return ITCPD_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int itcpd_sketch_candidates(int32_t nsk, int32_t D, int32_t K, int32_t ncand, const double* S,
+                            int32_t mem, float* out, int32_t device) {
+    if (!S || !out) return fail(ITCPD_E_ARG, "null argument");
+    if (nsk < 1 || D < 1 || D > 256 || K < 1 || K > 64 || ncand < 1 || ncand > K)
+        return fail(ITCPD_E_ARG, "need nsk >= 1, 1 <= D <= 256, 1 <= ncand <= K <= 64");
+    DeviceGuard dg(device);
+    hipStream_t st;
+    ITCPD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    double* Sd = nullptr;
+    float* od = nullptr;
+    const size_t ns = (size_t)nsk * D * K, no = (size_t)nsk * ncand * D;
+    int rc = ITCPD_OK;
+    if (mem == ITCPD_MEM_HOST) {
+        rc = dmalloc(&Sd, ns);
+        if (!rc) rc = dmalloc(&od, no);
+        if (!rc) rc = copy_in(Sd, S, ns * 8, ITCPD_MEM_HOST, st);
+    }
+    if (!rc) rc = launch_spa_candidates(nsk, D, K, ncand, Sd ? Sd : S, od ? od : out, st);
+    if (!rc && od) rc = copy_out(out, od, no * 4, ITCPD_MEM_HOST, st);
+    const hipError_t e = hipStreamSynchronize(st);
+    dfree(Sd);
+    dfree(od);
+    (void)hipStreamDestroy(st);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     return ITCPD_OK;
 }
 

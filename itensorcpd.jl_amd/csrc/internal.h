@@ -154,6 +154,81 @@ int launch_col_sumsq(itcpd_plan* p, int mode, const double* X);
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+
+// ------------------------------------------------------------ host helpers
+// Makes `dev` current for the scope and restores the caller's device after.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+template <typename T>
+inline int dmalloc(T** ptr, size_t count) {
+    *ptr = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)ptr, count * sizeof(T));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    return ITCPD_OK;
+}
+
+inline void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+inline int copy_in(void* dst, const void* src, size_t bytes, int mem, hipStream_t st) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, mem == ITCPD_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                                                 : hipMemcpyHostToDevice, st);
+    return e == hipSuccess ? ITCPD_OK : hip_fail(e, "hipMemcpyAsync(in)");
+}
+inline int copy_out(void* dst, const void* src, size_t bytes, int mem, hipStream_t st) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, mem == ITCPD_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                                                 : hipMemcpyDeviceToHost, st);
+    return e == hipSuccess ? ITCPD_OK : hip_fail(e, "hipMemcpyAsync(out)");
+}
+
+// ------------------------------------------------------------ warm-start MLP
+constexpr int MLP_MAX_LAYERS = 6;
+
+struct MlpDev {
+    int nlayers = 0;
+    int widths[MLP_MAX_LAYERS + 1] = {0};
+    int64_t nparams = 0;
+    float* params = nullptr;    // W_0 | b_0 | W_1 | b_1 | ...  (W_l: widths[l] x widths[l+1] row-major)
+    float* grads = nullptr;     // same layout
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    float* W[MLP_MAX_LAYERS] = {nullptr};
+    float* b[MLP_MAX_LAYERS] = {nullptr};
+    float* gW[MLP_MAX_LAYERS] = {nullptr};
+    float* gb[MLP_MAX_LAYERS] = {nullptr};
+    float* act[MLP_MAX_LAYERS] = {nullptr};   // batch_cap x widths[l+1]
+    float* dA = nullptr;                      // batch_cap x max width (two gradient buffers)
+    float* dB = nullptr;
+    double* loss_part = nullptr;
+    double* loss = nullptr;
+    int batch_cap = 0;
+    int64_t step = 0;
+};
+
+int mlp_gemm(hipStream_t st, bool ta, bool tb, int M, int N, int K, float alpha, const float* A, int lda,
+             const float* B, int ldb, float beta, float* C, int ldc);
+int mlp_forward(const MlpDev& m, int batch, const float* X, hipStream_t st);
+int mlp_backward(const MlpDev& m, int batch, const float* X, const float* Yt, const float* mask,
+                 double* loss_dev, hipStream_t st);
+int mlp_adam(MlpDev& m, float lr, hipStream_t st);
+int launch_spa_candidates(int nsk, int D, int K, int ncand, const double* S, float* out, hipStream_t st);
+
+// reconstruct.jl:2-9 on the device (+ optional N(0, sigma^2) noise, Philox stream)
+int launch_reconstruct(int order, const int64_t* dims, int64_t rank, const double* const* factors_dev,
+                       const double* lambda_dev, void* out, int out_elt, double sigma, uint64_t seed,
+                       hipStream_t st);
+
 // ----------------------------------------------------------------- batches
 constexpr int BATCH_MAXD = 96;   // largest mode size of a batched problem
 constexpr int BATCH_MAXR = 32;   // largest rank of a batched problem

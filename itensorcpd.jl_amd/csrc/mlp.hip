@@ -1,0 +1,293 @@
+// The warm-start network: a fully connected MLP (ReLU hidden layers, linear
+// output) trained and evaluated with the kernels below, FP32 throughout.
+//   forward  : H_{l+1} = act(H_l W_l + b_l)        (row-major: batch x width)
+//   backward : dH_l = (dH_{l+1} .* act'(H_{l+1})) W_l^T,  dW_l = H_l^T dZ_l,  db_l = 1^T dZ_l
+//   loss     : 0.5 * sum(mask .* (Y - Y*)^2) / batch
+//   update   : Adam (beta1 0.9, beta2 0.999, eps 1e-8)
+// No reference counterpart: the reference initialises from random_CPD only
+// (cpd.jl:63-82).  The network is what the learned warm start proposes from the
+// mode sketches; the ALS fit decides whether its proposal or random_CPD is used.
+#include "internal.h"
+
+#include <cmath>
+#include <vector>
+
+namespace itcpd {
+namespace {
+
+constexpr int TB = 64;    // output tile (TB x TB) per workgroup
+constexpr int TKB = 16;   // reduction slab
+constexpr int GT = 256;   // threads: 16 x 16, each a 4 x 4 micro-tile
+
+// C(MxN) = alpha * op(A)(MxK) * op(B)(KxN) + beta * C, all row-major.
+// op(A) = A (lda) or A^T (A stored KxM); op(B) likewise.  LDS tiles padded by
+// one column against bank conflicts; every global access is bounds-checked.
+template <bool TA, bool TBt>
+__global__ __launch_bounds__(GT) void k_sgemm(int M, int N, int K, float alpha,
+                                              const float* __restrict__ A, int lda,
+                                              const float* __restrict__ B, int ldb, float beta,
+                                              float* __restrict__ C, int ldc) {
+    __shared__ float As[TKB][TB + 1];
+    __shared__ float Bs[TKB][TB + 1];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int m0 = blockIdx.y * TB, n0 = blockIdx.x * TB;
+    float acc[4][4] = {};
+    for (int k0 = 0; k0 < K; k0 += TKB) {
+        for (int e = threadIdx.x; e < TKB * TB; e += GT) {
+            // A tile element (kk, mm) = op(A)(m0 + mm, k0 + kk)
+            const int kk = TA ? e / TB : e % TKB;
+            const int mm = TA ? e % TB : e / TKB;
+            const int m = m0 + mm, k = k0 + kk;
+            float v = 0.f;
+            if (m < M && k < K) v = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
+            As[kk][mm] = v;
+            // B tile element (kk, nn) = op(B)(k0 + kk, n0 + nn)
+            const int kb = TBt ? e % TKB : e / TB;
+            const int nb = TBt ? e / TKB : e % TB;
+            const int n = n0 + nb, k2 = k0 + kb;
+            float w = 0.f;
+            if (n < N && k2 < K) w = TBt ? B[(size_t)n * ldb + k2] : B[(size_t)k2 * ldb + n];
+            Bs[kb][nb] = w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TKB; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = m0 + ty + 16 * i;
+        if (m >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = n0 + tx + 16 * j;
+            if (n >= N) continue;
+            float* c = C + (size_t)m * ldc + n;
+            *c = alpha * acc[i][j] + (beta == 0.f ? 0.f : beta * *c);
+        }
+    }
+}
+
+// H = act(H + b) in place (row-major, width N)
+__global__ __launch_bounds__(256) void k_bias_act(float* __restrict__ H, const float* __restrict__ b,
+                                                  int64_t n, int N, int relu) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float v = H[i] + b[i % N];
+    H[i] = (relu && v < 0.f) ? 0.f : v;
+}
+
+// dZ = mask .* (Y - Y*) / batch (output layer), and the per-block loss partials
+__global__ __launch_bounds__(256) void k_mse_grad(const float* __restrict__ Y, const float* __restrict__ Yt,
+                                                  const float* __restrict__ mask, int64_t n, float inv_batch,
+                                                  float* __restrict__ dZ, double* __restrict__ loss_part) {
+    __shared__ double red[256];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double l = 0.0;
+    if (i < n) {
+        const float mk = mask ? mask[i] : 1.f;
+        const float d = mk * (Y[i] - Yt[i]);
+        dZ[i] = d * inv_batch;
+        l = 0.5 * (double)d * (double)d * (double)inv_batch;
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+// dZ .*= (H > 0)  (ReLU derivative from the activation output)
+__global__ __launch_bounds__(256) void k_relu_bwd(float* __restrict__ dZ, const float* __restrict__ H, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n && !(H[i] > 0.f)) dZ[i] = 0.f;
+}
+
+// db(j) = sum_i dZ(i, j): one thread per column, rows in order
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ dZ, int rows, int N, float* __restrict__ db) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= N) return;
+    float s = 0.f;
+    for (int i = 0; i < rows; ++i) s += dZ[(size_t)i * N + j];
+    db[j] = s;
+}
+
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, int64_t n,
+                                              float lr, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi;
+    const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
+__global__ __launch_bounds__(256) void k_sum_loss(const double* __restrict__ part, int n, double* __restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// Candidate factor columns of one mode from its sketch S (D x K, column-major),
+// one workgroup per sketch: `ncand` rounds of successive projection -- take the
+// row of largest norm, d = that row / its norm, candidate c = S d (positive at
+// the chosen row), deflate S <- S - c d^T.  Rounds pick components in order of
+// sketch energy (lambda_r^2 up to the sketch's random scaling), the same order in
+// every mode.  Output rows: candidate k of sketch s at out[(s * ncand + k) * D],
+// scaled to unit maximum magnitude.  D <= 256, K <= 64 (the sketch in LDS).
+__global__ __launch_bounds__(256) void k_spa_candidates(const double* __restrict__ S, int D, int K,
+                                                        int ncand, float* __restrict__ out) {
+    __shared__ double Sl[256 * 65];
+    __shared__ double d[64];
+    __shared__ double redv[256];
+    __shared__ int redi[256];
+    __shared__ double cmax[256];
+    const int t = threadIdx.x;
+    const double* Sg = S + (size_t)blockIdx.x * D * K;
+    for (int x = t; x < D * K; x += 256) {
+        const int i = x % D, c = x / D;
+        Sl[i * 65 + c] = Sg[x];
+    }
+    __syncthreads();
+    for (int k = 0; k < ncand; ++k) {
+        double nr = -1.0;
+        if (t < D) {
+            nr = 0.0;
+            for (int c = 0; c < K; ++c) nr += Sl[t * 65 + c] * Sl[t * 65 + c];
+        }
+        redv[t] = nr;
+        redi[t] = t;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o) {
+                const double v2 = redv[t + o];
+                const int i2 = redi[t + o];
+                if (v2 > redv[t] || (v2 == redv[t] && i2 < redi[t])) { redv[t] = v2; redi[t] = i2; }
+            }
+            __syncthreads();
+        }
+        const int istar = redi[0];
+        const double nrm = sqrt(redv[0]);
+        __syncthreads();
+        if (t < K) d[t] = nrm > 0.0 ? Sl[istar * 65 + t] / nrm : 0.0;
+        __syncthreads();
+        double cv = 0.0;
+        if (t < D) {
+            for (int c = 0; c < K; ++c) cv += Sl[t * 65 + c] * d[c];
+            for (int c = 0; c < K; ++c) Sl[t * 65 + c] -= cv * d[c];
+        }
+        cmax[t] = fabs(cv);
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o) cmax[t] = fmax(cmax[t], cmax[t + o]);
+            __syncthreads();
+        }
+        const double scale = cmax[0] > 0.0 ? 1.0 / cmax[0] : 0.0;
+        if (t < D) out[((size_t)blockIdx.x * ncand + k) * D + t] = (float)(cv * scale);
+        __syncthreads();
+    }
+}
+
+unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+int mlp_gemm(hipStream_t st, bool ta, bool tb, int M, int N, int K, float alpha, const float* A, int lda,
+             const float* B, int ldb, float beta, float* C, int ldc) {
+    dim3 grid((unsigned)((N + TB - 1) / TB), (unsigned)((M + TB - 1) / TB));
+    if (!ta && !tb) hipLaunchKernelGGL((k_sgemm<false, false>), grid, dim3(GT), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (ta && !tb) hipLaunchKernelGGL((k_sgemm<true, false>), grid, dim3(GT), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    else if (!ta && tb) hipLaunchKernelGGL((k_sgemm<false, true>), grid, dim3(GT), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    else hipLaunchKernelGGL((k_sgemm<true, true>), grid, dim3(GT), 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+// ---------------------------------------------------------------- network
+int mlp_forward(const MlpDev& m, int batch, const float* X, hipStream_t st) {
+    const float* in = X;
+    for (int l = 0; l < m.nlayers; ++l) {
+        const int K = m.widths[l], N = m.widths[l + 1];
+        float* H = m.act[l];
+        int rc;
+        if ((rc = mlp_gemm(st, false, false, batch, N, K, 1.f, in, K, m.W[l], N, 0.f, H, N))) return rc;
+        const int64_t n = (int64_t)batch * N;
+        hipLaunchKernelGGL(k_bias_act, dim3(nblk(n)), dim3(256), 0, st, H, m.b[l], n, N,
+                           l + 1 < m.nlayers ? 1 : 0);
+        ITCPD_HIP(hipGetLastError());
+        in = H;
+    }
+    return ITCPD_OK;
+}
+
+// Gradients of the loss for one batch into m.gW / m.gb; loss -> *loss_dev (double)
+int mlp_backward(const MlpDev& m, int batch, const float* X, const float* Yt, const float* mask,
+                 double* loss_dev, hipStream_t st) {
+    const int L = m.nlayers;
+    const int Nout = m.widths[L];
+    const int64_t nout = (int64_t)batch * Nout;
+    float* dZ = m.dA;
+    hipLaunchKernelGGL(k_mse_grad, dim3(nblk(nout)), dim3(256), 0, st, m.act[L - 1], Yt, mask, nout,
+                       1.f / (float)batch, dZ, m.loss_part);
+    ITCPD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(256), 0, st, m.loss_part, (int)nblk(nout), loss_dev);
+    ITCPD_HIP(hipGetLastError());
+    int rc;
+    for (int l = L - 1; l >= 0; --l) {
+        const int K = m.widths[l], N = m.widths[l + 1];
+        const float* in = l == 0 ? X : m.act[l - 1];
+        // dW_l = in^T dZ, db_l = colsum(dZ)
+        if ((rc = mlp_gemm(st, true, false, K, N, batch, 1.f, in, K, dZ, N, 0.f, m.gW[l], N))) return rc;
+        hipLaunchKernelGGL(k_colsum, dim3(nblk(N)), dim3(256), 0, st, dZ, batch, N, m.gb[l]);
+        ITCPD_HIP(hipGetLastError());
+        if (l == 0) break;
+        // dZ_{l-1} = (dZ W_l^T) .* relu'(act[l-1])
+        float* dPrev = (dZ == m.dA) ? m.dB : m.dA;
+        if ((rc = mlp_gemm(st, false, true, batch, K, N, 1.f, dZ, N, m.W[l], N, 0.f, dPrev, K))) return rc;
+        const int64_t np = (int64_t)batch * K;
+        hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(np)), dim3(256), 0, st, dPrev, m.act[l - 1], np);
+        ITCPD_HIP(hipGetLastError());
+        dZ = dPrev;
+    }
+    return ITCPD_OK;
+}
+
+int launch_spa_candidates(int nsk, int D, int K, int ncand, const double* S, float* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_spa_candidates, dim3((unsigned)nsk), dim3(256), 0, st, S, D, K, ncand, out);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+int mlp_adam(MlpDev& m, float lr, hipStream_t st) {
+    m.step += 1;
+    const float c1 = 1.f - powf(0.9f, (float)m.step);
+    const float c2 = 1.f - powf(0.999f, (float)m.step);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(m.nparams)), dim3(256), 0, st, m.params, m.grads, m.adam_m,
+                       m.adam_v, m.nparams, lr, c1, c2);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+}  // namespace itcpd

@@ -605,6 +605,44 @@ __global__ __launch_bounds__(256) void k_random_normal(double* __restrict__ X, i
     const double u2 = (double)(b >> 11) * 1.1102230246251565e-16;        // [0, 1)
     X[e] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
+// reconstruct.jl:2-9: X(i_1..i_N) = sum_r lambda_r prod_n A_n(i_n, r), column-major,
+// one element per thread, plus sigma * N(0,1) from the Philox stream keyed by
+// (seed, element index) when sigma > 0.  Written as FP64 or FP32.
+struct RecoArgs {
+    const double* A[ITCPD_MAX_ORDER];
+    int64_t dims[ITCPD_MAX_ORDER];
+    int N;
+};
+template <typename OT>
+__global__ __launch_bounds__(256) void k_reconstruct(RecoArgs ra, int R, const double* __restrict__ lam,
+                                                     int64_t nel, double sigma, uint64_t seed,
+                                                     OT* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= nel) return;
+    int64_t idx[ITCPD_MAX_ORDER];
+    int64_t t = e;
+    for (int n = 0; n < ra.N; ++n) {
+        idx[n] = t % ra.dims[n];
+        t /= ra.dims[n];
+    }
+    double x = 0.0;
+    for (int r = 0; r < R; ++r) {
+        double pr = lam[r];
+        for (int n = 0; n < ra.N; ++n) pr *= ra.A[n][idx[n] + ra.dims[n] * r];
+        x += pr;
+    }
+    if (sigma > 0.0) {
+        uint32_t c[4] = {(uint32_t)e, (uint32_t)((uint64_t)e >> 32), 0x6E6F6973u, 0u};
+        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const uint64_t a = ((uint64_t)c[0] << 32) | c[1];
+        const uint64_t b = ((uint64_t)c[2] << 32) | c[3];
+        const double u1 = (double)((a >> 11) + 1) * 1.1102230246251565e-16;
+        const double u2 = (double)(b >> 11) * 1.1102230246251565e-16;
+        x += sigma * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
+    out[e] = (OT)x;
+}
+
 // per-block sums of squares of the columns of X (for the init's row_norm)
 __global__ __launch_bounds__(256) void k_col_sumsq(const double* __restrict__ X, int64_t In, int R,
                                                    int rows_per_block, double* __restrict__ part) {
@@ -902,6 +940,28 @@ int launch_col_sumsq(itcpd_plan* p, int mode, const double* X) {
     const int per = (int)cdiv(In, nsp);
     hipLaunchKernelGGL(k_col_sumsq, dim3(nsp), dim3(256), 0, p->stream, X, In, (int)p->R, per,
                        p->sumsq_part);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+
+int launch_reconstruct(int order, const int64_t* dims, int64_t rank, const double* const* factors_dev,
+                       const double* lambda_dev, void* out, int out_elt, double sigma, uint64_t seed,
+                       hipStream_t st) {
+    RecoArgs ra{};
+    ra.N = order;
+    int64_t nel = 1;
+    for (int n = 0; n < order; ++n) {
+        ra.A[n] = factors_dev[n];
+        ra.dims[n] = dims[n];
+        nel *= dims[n];
+    }
+    if (out_elt == 8)
+        hipLaunchKernelGGL(k_reconstruct<double>, dim3((unsigned)cdiv(nel, 256)), dim3(256), 0, st, ra,
+                           (int)rank, lambda_dev, nel, sigma, seed, (double*)out);
+    else
+        hipLaunchKernelGGL(k_reconstruct<float>, dim3((unsigned)cdiv(nel, 256)), dim3(256), 0, st, ra,
+                           (int)rank, lambda_dev, nel, sigma, seed, (float*)out);
     ITCPD_HIP(hipGetLastError());
     return ITCPD_OK;
 }

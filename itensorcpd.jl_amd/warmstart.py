@@ -1,0 +1,190 @@
+"""Learned warm start for CP-ALS (no reference counterpart: the reference always
+starts from random_CPD, cpd.jl:63-82, and leaves the rest to als_optimize).
+
+Pipeline, all on the device:
+  1. mode sketches S_n = T_(n) Omega_n with Omega_n the Khatri-Rao product of fixed
+     seeded Gaussian matrices (I_m x K) of the other modes -- i.e. the MTTKRP of T
+     with K random "factors", computed by the engine's own MTTKRP kernels;
+  2. candidate columns per mode by successive projection on S_n
+     (itcpd_sketch_candidates): rounds take components in order of sketch energy,
+     which is the same order in every mode, so candidate k of each mode belongs to
+     one component;
+  3. the network (MLP, ReLU, trained with the engine's HIP kernels) maps each
+     candidate column (I values, unit max) to a cleaned factor column;
+  4. the proposal (columns normalised, lambda = 1) is judged by its FP64 fit
+     against a random_CPD draw: the better start is used ("ALS fit as the judge",
+     random_CPD as the fallback).
+
+Training pairs come from ALS itself: synthetic tensors (Gaussian-bump factors,
+log-uniform lambda, noise at a fixed SNR) are decomposed by the FP64 engine
+starting from their generating factors; each candidate column is paired with the
+converged factor column it matches best.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+from .api import CPD, ALSPlan, DeviceTensor, FitCheck, MLP, random_CPD, reconstruct
+
+
+# ------------------------------------------------------- synthetic tensors
+def bump_factors(I: int, R: int, rng: np.random.Generator) -> np.ndarray:
+    """I x R columns exp(-(x - mu)^2 / (2 w^2)) on x = linspace(0, 1, I), mu ~ U(0, 1),
+    w ~ U(0.02, 0.15)."""
+    x = np.linspace(0.0, 1.0, I)[:, None]
+    mu = rng.uniform(0.0, 1.0, R)[None, :]
+    w = rng.uniform(0.02, 0.15, R)[None, :]
+    return np.exp(-((x - mu) ** 2) / (2.0 * w * w))
+
+
+def bump_problem(dims, R: int, rng: np.random.Generator, snr_db: float = 30.0):
+    """Factors, lambda (log-uniform in [0.1, 10]) and the noise sigma for the SNR."""
+    fac = [bump_factors(d, R, rng) for d in dims]
+    lam = np.exp(rng.uniform(np.log(0.1), np.log(10.0), R))
+    G = np.ones((R, R))
+    for f in fac:
+        G *= f.T @ f
+    signal = float(np.sqrt(max(lam @ G @ lam, 0.0)))
+    sigma = signal / (10 ** (snr_db / 20.0)) / np.sqrt(float(np.prod(dims)))
+    return fac, lam, sigma
+
+
+# ---------------------------------------------------------------- the model
+class WarmStart:
+    """Sketch -> candidates -> network -> proposal for order-3 tensors of fixed dims."""
+
+    def __init__(self, dims, rmax: int = 24, sketch_rank: int = 48, hidden=(256, 256),
+                 seed: int = 0, batch_cap: int = 512, device: int = 0):
+        self.dims = tuple(int(d) for d in dims)
+        if len(self.dims) != 3 or len(set(self.dims)) != 1:
+            raise ValueError("the warm start handles cubic order-3 tensors")
+        self.D = self.dims[0]
+        self.K = int(sketch_rank)
+        self.rmax = int(rmax)
+        rng = np.random.default_rng(10_000 + seed)
+        self.omega = [rng.standard_normal((d, self.K)) for d in self.dims]
+        self.mlp = MLP([self.D, *hidden, self.D], batch_cap=batch_cap, seed=seed, device=device)
+        self.device = device
+        self._plans = {}
+
+    # -- device plans bound to a DeviceTensor, cached per (device pointer, rank);
+    # release() drops them (call it before freeing the tensor)
+    def _plan(self, T, R: int, key=None) -> ALSPlan:
+        if not isinstance(T, DeviceTensor):
+            raise TypeError("the warm start works on a DeviceTensor (bound without a copy)")
+        k = (T.ptr, R)
+        p = self._plans.get(k)
+        if p is None:
+            p = ALSPlan(T, R)
+            self._plans[k] = p
+        return p
+
+    def release(self):
+        for p in self._plans.values():
+            p.close()
+        self._plans = {}
+
+    def sketches(self, T, key=None) -> np.ndarray:
+        """(3, D, K) sketches T_(n) (KRP of the other Omegas)."""
+        p = self._plan(T, self.K, key)
+        p.set_cpd(CPD(self.omega, np.ones(self.K)))
+        return np.stack([p.mttkrp(n) for n in range(3)])
+
+    def candidates(self, S: np.ndarray, R: int) -> np.ndarray:
+        """(3 R, D) candidate columns (unit max), mode-major."""
+        S = np.ascontiguousarray(np.stack([np.asfortranarray(s).ravel(order="F") for s in S]))
+        out = np.empty((3 * R, self.D), dtype=np.float32)
+        L.check(L.load().itcpd_sketch_candidates(3, self.D, self.K, int(R), C.c_void_p(S.ctypes.data),
+                                                 L.ITCPD_MEM_HOST, C.c_void_p(out.ctypes.data), self.device))
+        return out
+
+    def propose(self, T, R: int, key=None, use_network: bool = True) -> CPD:
+        cand = self.candidates(self.sketches(T, key), R)
+        cols = self.mlp.forward(cand) if use_network else cand
+        fac = []
+        for n in range(3):
+            A = np.asfortranarray(cols[n * R:(n + 1) * R].T.astype(np.float64))
+            nrm = np.linalg.norm(A, axis=0)
+            nrm[nrm == 0] = 1.0
+            fac.append(np.asfortranarray(A / nrm))
+        return CPD(fac, np.ones(R))
+
+    def fit_of(self, T, cp: CPD, nrm: float, key=None) -> float:
+        """FP64 fit of a CPD against T (fit_check.jl:24-32 via the engine's hooks)."""
+        p = self._plan(T, cp.rank, key)
+        p.set_cpd(cp)
+        return p.fit(p.mttkrp(2), nrm)[2]
+
+    def init(self, T, R: int, rng=None, key=None):
+        """The judged start: (CPD, "network" | "random_CPD")."""
+        nrm = float(self._plan(T, R, key).tensor_norm())
+        prop = self.propose(T, R, key)
+        rnd = random_CPD(self.dims, R, rng=rng)
+        fp, fr = self.fit_of(T, prop, nrm, key), self.fit_of(T, rnd, nrm, key)
+        return (prop, "network") if fp >= fr else (rnd, "random_CPD")
+
+
+# ------------------------------------------------------------ training data
+def training_pairs(ws: WarmStart, n_tensors: int, seed: int, rlo: int = 8, rhi: int = 24,
+                   snr_db: float = 30.0, log=None):
+    """(inputs, targets): candidate columns and the converged factor columns they
+    match (|cosine|, greedy), sign-aligned and scaled to unit maximum."""
+    rng = np.random.default_rng(seed)
+    T = DeviceTensor(ws.dims, np.float32, ws.device)
+    key = ("train", T.ptr)
+    X, Y = [], []
+    t0 = time.perf_counter()
+    for it in range(n_tensors):
+        R = int(rng.integers(rlo, rhi + 1))
+        fac, lam, sigma = bump_problem(ws.dims, R, rng, snr_db)
+        reconstruct(CPD(fac, lam), out=T, noise_sigma=sigma, seed=int(rng.integers(1 << 62)))
+        p = ws._plan(T, R, key)
+        nrm = p.tensor_norm()
+        p.set_cpd(CPD([np.asfortranarray(f / np.linalg.norm(f, axis=0)) for f in fac],
+                      lam * np.prod([np.linalg.norm(f, axis=0) for f in fac], axis=0)))
+        p.run(FitCheck(1e-10, 30, nrm))
+        conv = p.get_cpd()
+        cand = ws.candidates(ws.sketches(T, key), R)
+        for n in range(3):
+            A = conv.factors[n]
+            An = A / np.maximum(np.linalg.norm(A, axis=0), 1e-300)
+            Cn = cand[n * R:(n + 1) * R].astype(np.float64)
+            cn = Cn / np.maximum(np.linalg.norm(Cn, axis=1, keepdims=True), 1e-300)
+            cos = cn @ An                        # candidates x components
+            used = np.zeros(R, dtype=bool)
+            for k in np.argsort(-np.abs(cos).max(axis=1)):
+                row = np.where(used, -1.0, np.abs(cos[k]))
+                r = int(np.argmax(row))
+                used[r] = True
+                col = An[:, r] * np.sign(cos[k, r] if cos[k, r] != 0 else 1.0)
+                X.append(Cn[k])
+                Y.append(col / np.abs(col).max())
+        if log and (it + 1) % max(1, n_tensors // 10) == 0:
+            log(f"[warmstart] {it + 1}/{n_tensors} training tensors ({time.perf_counter() - t0:.1f} s)")
+    ws.release()
+    T.free()
+    return np.asarray(X, dtype=np.float32), np.asarray(Y, dtype=np.float32)
+
+
+def train(ws: WarmStart, X: np.ndarray, Y: np.ndarray, epochs: int = 30, batch: int = 256,
+          lr: float = 1e-3, seed: int = 0, log=None):
+    """Adam on the engine's MLP kernels; returns the per-epoch mean loss."""
+    rng = np.random.default_rng(seed)
+    hist = []
+    n = X.shape[0]
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        tot, cnt = 0.0, 0
+        for b0 in range(0, n - batch + 1, batch):
+            idx = order[b0:b0 + batch]
+            tot += ws.mlp.train_step(X[idx], Y[idx], lr=lr)
+            cnt += 1
+        hist.append(tot / max(cnt, 1))
+        if log and (ep + 1) % max(1, epochs // 5) == 0:
+            log(f"[warmstart] epoch {ep + 1}: loss {hist[-1]:.4e}")
+    return hist

@@ -1,0 +1,79 @@
+"""CPU restatement of the warm-start network and its input transform (test
+infrastructure only: checks the HIP kernels, never part of the product path).
+
+* mlp_forward / mlp_loss_grad: the fully connected ReLU network with a linear
+  output and the masked squared loss 0.5 * sum(mask (Y - Y*)^2) / batch, in float64,
+  with the parameter layout W_0 | b_0 | W_1 | b_1 ... (W_l: widths[l] x widths[l+1],
+  row-major).  Pinned by central finite differences (tests/test_oracle.py).
+* spa_candidates: successive projection on a mode sketch (largest-norm row ->
+  direction, candidate = S d, deflate), pinned by planted separable inputs.
+
+The reference has no counterpart (it initialises from random_CPD, cpd.jl:63-82).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def unpack(params, widths):
+    Ws, bs, off = [], [], 0
+    for l in range(len(widths) - 1):
+        k, n = widths[l], widths[l + 1]
+        Ws.append(np.asarray(params[off:off + k * n], dtype=np.float64).reshape(k, n))
+        off += k * n
+        bs.append(np.asarray(params[off:off + n], dtype=np.float64))
+        off += n
+    return Ws, bs
+
+
+def mlp_forward(params, widths, X, return_acts=False):
+    Ws, bs = unpack(params, widths)
+    H = np.asarray(X, dtype=np.float64)
+    acts = [H]
+    for l, (W, b) in enumerate(zip(Ws, bs)):
+        H = H @ W + b
+        if l + 1 < len(Ws):
+            H = np.maximum(H, 0.0)
+        acts.append(H)
+    return (H, acts) if return_acts else H
+
+
+def mlp_loss_grad(params, widths, X, Yt, mask=None):
+    """(loss, gradient) of 0.5 * sum(mask (Y - Y*)^2) / batch."""
+    Ws, _ = unpack(params, widths)
+    B = X.shape[0]
+    Y, acts = mlp_forward(params, widths, X, return_acts=True)
+    mk = np.ones_like(Y) if mask is None else np.asarray(mask, dtype=np.float64)
+    D = mk * (Y - np.asarray(Yt, dtype=np.float64))
+    loss = 0.5 * float(np.sum(D * D)) / B
+    dZ = D / B
+    grads = []
+    for l in range(len(Ws) - 1, -1, -1):
+        H = acts[l]
+        gW = H.T @ dZ
+        gb = dZ.sum(axis=0)
+        grads.append((gW, gb))
+        if l == 0:
+            break
+        dZ = (dZ @ Ws[l].T) * (acts[l] > 0)
+    flat = []
+    for gW, gb in reversed(grads):
+        flat.append(gW.ravel())
+        flat.append(gb)
+    return loss, np.concatenate(flat)
+
+
+def spa_candidates(S, ncand):
+    """Rows: candidate k, scaled to unit maximum magnitude (D-vectors)."""
+    S = np.array(S, dtype=np.float64)
+    out = np.zeros((ncand, S.shape[0]))
+    for k in range(ncand):
+        nr = np.sum(S * S, axis=1)
+        i = int(np.argmax(nr))
+        nrm = np.sqrt(nr[i])
+        d = S[i] / nrm if nrm > 0 else np.zeros(S.shape[1])
+        c = S @ d
+        S = S - np.outer(c, d)
+        m = np.abs(c).max()
+        out[k] = c / m if m > 0 else c
+    return out

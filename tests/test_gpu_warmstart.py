@@ -1,0 +1,140 @@
+"""The learned warm start: its HIP kernels (network forward / backward / Adam,
+successive-projection candidates, device reconstruct) against the CPU oracle, and
+what the trained network gains over random_CPD on held-out tensors."""
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import warmstart as W
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("widths,batch", [([20, 32, 16, 12], 10), ([65, 130, 7], 1), ([200, 256, 256, 200], 96)])
+def test_network_forward_and_gradients_match_oracle(itcpd, widths, batch):
+    rng = np.random.default_rng(len(widths) * 100 + batch)
+    with itcpd.MLP(widths, batch_cap=128, seed=3) as net:
+        p = net.params().astype(np.float64)
+        # biases start at zero; give them values so the bias paths are exercised
+        p = p + 0.05 * rng.standard_normal(p.size)
+        net.set_params(p)
+        X = rng.standard_normal((batch, widths[0])).astype(np.float32)
+        Yt = rng.standard_normal((batch, widths[-1])).astype(np.float32)
+        mask = (rng.random((batch, widths[-1])) > 0.25).astype(np.float32)
+        Xd = X.astype(np.float64)
+        want = W.mlp_forward(net.params(), widths, Xd)
+        assert rel(net.forward(X), want) < 1e-5
+        g, loss = net.gradients(X, Yt, mask)
+        lw, gw = W.mlp_loss_grad(net.params(), widths, Xd, Yt, mask)
+        assert abs(loss - lw) < 1e-5 * abs(lw)
+        assert rel(g, gw) < 1e-4
+
+
+def test_adam_step_matches_oracle(itcpd):
+    rng = np.random.default_rng(7)
+    widths = [12, 16, 8]
+    with itcpd.MLP(widths, batch_cap=32, seed=5) as net:
+        p0 = net.params().astype(np.float64)
+        X = rng.standard_normal((32, 12)).astype(np.float32)
+        Yt = rng.standard_normal((32, 8)).astype(np.float32)
+        _, g = W.mlp_loss_grad(p0, widths, X.astype(np.float64), Yt)
+        lr = 1e-2
+        net.train_step(X, Yt, lr=lr)
+        m = 0.1 * g
+        v = 0.001 * g * g
+        want = p0 - lr * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+        assert np.abs(net.params() - want).max() < 1e-5
+
+
+def test_training_reduces_loss(itcpd):
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((512, 24)).astype(np.float32)
+    Wtrue = rng.standard_normal((24, 10))
+    Y = np.tanh(X @ Wtrue).astype(np.float32)
+    with itcpd.MLP([24, 64, 64, 10], batch_cap=128, seed=1) as net:
+        first = last = None
+        for it in range(300):
+            idx = rng.choice(512, 128, replace=False)
+            loss = net.train_step(X[idx], Y[idx], lr=3e-3)
+            first = loss if first is None else first
+            last = loss
+        assert last < 0.2 * first
+
+
+def test_sketch_candidates_match_oracle(itcpd):
+    rng = np.random.default_rng(9)
+    D, K, n = 200, 48, 24
+    S = [rng.standard_normal((D, K)) @ np.diag(np.linspace(3, 0.5, K)) for _ in range(3)]
+    ws_like = np.stack([s.ravel(order="F") for s in S])
+    import ctypes as C
+    out = np.empty((3 * n, D), dtype=np.float32)
+    L = itcpd._lib
+    L.check(L.load().itcpd_sketch_candidates(3, D, K, n, C.c_void_p(np.ascontiguousarray(ws_like).ctypes.data),
+                                             L.ITCPD_MEM_HOST, C.c_void_p(out.ctypes.data), 0))
+    for s in range(3):
+        want = W.spa_candidates(S[s], n)
+        assert np.abs(out[s * n:(s + 1) * n] - want).max() < 1e-5
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-6)])
+def test_device_reconstruct_matches_oracle(itcpd, dtype, tol):
+    rng = np.random.default_rng(10)
+    dims, R = (17, 12, 9), 5
+    fac = [rng.standard_normal((d, R)) for d in dims]
+    lam = rng.uniform(0.5, 2.0, R)
+    T = itcpd.reconstruct(itcpd.CPD(fac, lam), dtype=dtype).to_numpy()
+    assert rel(T, O.reconstruct(fac, lam)) < tol
+    # noise: N(0, sigma^2) from the Philox stream, reproducible by seed
+    big = [rng.standard_normal((d, 2)) for d in (60, 50, 40)]
+    Tn = itcpd.reconstruct(itcpd.CPD(big, np.ones(2)), noise_sigma=0.5, seed=11).to_numpy()
+    noise = Tn - O.reconstruct(big, np.ones(2))
+    assert abs(noise.std() - 0.5) < 0.01 and abs(noise.mean()) < 0.01
+    Tn2 = itcpd.reconstruct(itcpd.CPD(big, np.ones(2)), noise_sigma=0.5, seed=11).to_numpy()
+    assert np.array_equal(Tn, Tn2)
+
+
+def fit_trace(itcpd, T, cp, max_sweeps, tol=0.0):
+    nrm = float(np.linalg.norm(T.to_numpy().astype(np.float64)))
+    with itcpd.ALSPlan(T, cp.rank) as plan:
+        plan.set_cpd(cp)
+        chk = itcpd.FitCheck(tol, max_sweeps, nrm)
+        info = plan.run(chk)
+    return np.asarray(info["fits"])
+
+
+def first_hit(fits, target):
+    hit = np.flatnonzero(fits >= target)
+    return int(hit[0]) + 1 if hit.size else len(fits) + 1
+
+
+def test_trained_network_gains_over_random_init(itcpd):
+    """Bump tensors (64^3, R in [8, 16], 30 dB SNR): train on 150 tensors, then on 6
+    held-out tensors count ALS sweeps until the FP64 fit is within 1e-6 of fit*, the
+    converged plain-FP64 fit from random_CPD (FitCheck(1e-10, 1000)); from random_CPD
+    itself, from the untrained pipeline (candidates alone) and from the trained
+    network.  The network must need fewer sweeps than random_CPD."""
+    from itensorcpd_jl_amd import warmstart as WS
+    ws = WS.WarmStart((64, 64, 64), rmax=16, sketch_rank=32, hidden=(256, 256), seed=0)
+    X, Y = WS.training_pairs(ws, 150, seed=1, rlo=8, rhi=16)
+    hist = WS.train(ws, X, Y, epochs=40, batch=256, lr=2e-3)
+    assert hist[-1] < 0.5 * hist[0]
+    rng = np.random.default_rng(99)
+    res = {"random": [], "candidates": [], "network": []}
+    for t in range(6):
+        R = int(rng.integers(8, 17))
+        fac, lam, sigma = WS.bump_problem((64, 64, 64), R, rng)
+        T = itcpd.reconstruct(itcpd.CPD(fac, lam), dtype=np.float32, noise_sigma=sigma, seed=1000 + t)
+        fr = fit_trace(itcpd, T, itcpd.random_CPD(T.shape, R, rng=t), 1000, tol=1e-10)
+        target = fr[-1] - 1e-6
+        res["random"].append(first_hit(fr, target))
+        res["candidates"].append(first_hit(fit_trace(itcpd, T, ws.propose(T, R, use_network=False), 1000), target))
+        res["network"].append(first_hit(fit_trace(itcpd, T, ws.propose(T, R), 1000), target))
+        ws.release()
+        T.free()
+    ws.release()
+    print("sweeps to fit:", res)
+    assert np.median(res["network"]) < np.median(res["random"])

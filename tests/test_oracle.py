@@ -258,3 +258,42 @@ def test_cp_als_bar_exact_rank4():
     chk = O.FitCheck(1e-6, 100, float(np.linalg.norm(A)))
     out = O.decompose(A, 400, check=chk)
     assert np.linalg.norm(O.reconstruct(out.factors, out.lam) - A) / np.linalg.norm(A) < 1e-3
+
+
+# ------------------------------------------------------ warm-start oracle
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import warmstart as W
+    rng = np.random.default_rng(11)
+    widths = [7, 9, 6, 5]
+    n = sum(widths[l] * widths[l + 1] + widths[l + 1] for l in range(3))
+    p = rng.standard_normal(n) * 0.5
+    X, Yt = rng.standard_normal((4, 7)), rng.standard_normal((4, 5))
+    mask = (rng.random((4, 5)) > 0.3).astype(float)
+    loss, g = W.mlp_loss_grad(p, widths, X, Yt, mask)
+    h = 1e-6
+    fd = np.empty(n)
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = h
+        fd[k] = (W.mlp_loss_grad(p + e, widths, X, Yt, mask)[0] - W.mlp_loss_grad(p - e, widths, X, Yt, mask)[0]) / (2 * h)
+    assert np.abs(g - fd).max() < 1e-6 * max(1.0, np.abs(fd).max())
+    # the loss itself: 0.5 * sum(mask (Y - Y*)^2), ReLU network by hand for one row
+    Ws, bs = W.unpack(p, widths)
+    y = np.maximum(np.maximum(X[0] @ Ws[0] + bs[0], 0) @ Ws[1] + bs[1], 0) @ Ws[2] + bs[2]
+    np.testing.assert_allclose(W.mlp_forward(p, widths, X)[0], y, rtol=1e-12)
+
+
+def test_spa_candidates_recover_planted_columns():
+    from oracle import warmstart as W
+    # S = A Wt with A having disjoint supports and Wt orthonormal rows: the candidates
+    # are exactly the columns of A (scaled to unit max), in order of decreasing energy
+    rng = np.random.default_rng(12)
+    D, K, R = 30, 12, 4
+    A = np.zeros((D, R))
+    for r in range(R):
+        A[5 * r + 1: 5 * r + 5, r] = rng.uniform(0.5, 1.0, 4) * (R - r)
+    Wt = np.linalg.qr(rng.standard_normal((K, R)))[0].T
+    cand = W.spa_candidates(A @ Wt, R)
+    for k in range(R):
+        want = A[:, k] / np.abs(A[:, k]).max()
+        np.testing.assert_allclose(cand[k], want, atol=1e-12)
