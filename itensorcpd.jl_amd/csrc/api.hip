@@ -136,8 +136,9 @@ int alloc_factor_state(itcpd_plan* p) {
 // One ALS sweep (optimize.jl:17-30).  Mode n: V_n is factored, M_n computed, then
 // solve, row_norm, Gram refresh (and the fit after the last mode) run on the main
 // stream.  The first mode of each side gets M from a GEMM launch that fills every
-// CU, so its factorisation runs just before that GEMM; the other modes get M from
-// a short Hadamard-reduce and their factorisation runs beside it on the side stream.
+// CU, so its factorisation runs just before that GEMM on the main stream; the
+// other modes get M from a short Hadamard-reduce and their factorisation runs
+// beside it on the side stream.
 int sweep(itcpd_plan* p) {
     int rc;
     const int N = p->N;
@@ -248,6 +249,10 @@ int itcpd_plan_create(itcpd_plan** out, int32_t order, const int64_t* dims, int6
     }
     if ((rc = alloc_factor_state(p))) return bail(rc);
     if ((rc = dmalloc(&p->U, (size_t)(R * R)))) return bail(rc);
+    if (R <= 64) {
+        p->RT = R <= 16 ? 16 : (R <= 32 ? 32 : 64);
+        if ((rc = dmalloc(&p->Upad, (size_t)(R * R)))) return bail(rc);   // V^-1
+    }
     if ((rc = dmalloc(&p->invdiag, (size_t)R))) return bail(rc);
     if ((rc = dmalloc(&p->qr_aux, (size_t)(R * R + 4 * R)))) return bail(rc);
     if ((rc = dmalloc(&p->piv, (size_t)(2 * R)))) return bail(rc);
@@ -299,6 +304,7 @@ int itcpd_plan_destroy(itcpd_plan* p) {
     dfree(p->piv);
     dfree(p->finfo);
     dfree(p->Af);
+    dfree(p->Upad);
     dfree(p->PL);
     dfree(p->PR);
     dfree(p->parts);

@@ -185,6 +185,9 @@ __device__ void batch_mttkrp(const double* __restrict__ T, int I, int J, int K, 
 // Full rank: V is overwritten by U (row-major, pivoted order), diagU / piv set.
 __device__ void batch_cholesky(int R, double tol, const BatchLds& L) {
     const int t = threadIdx.x;
+    // zero UbT: entries t >= j of row l are read as zeros by the padded dot products
+    for (int x = t; x < BATCH_MAXR * LDT; x += blockDim.x) L.UbT[x] = 0.0;
+    __syncthreads();
     if (t < 64) {
         const int l = t;
         bool rem = l < R;
@@ -207,14 +210,21 @@ __device__ void batch_cholesky(int R, double tol, const BatchLds& L) {
             if (rem) {
                 const double* up = L.UbT + p * LDT;
                 const double* ul = L.UbT + l * LDT;
-                double acc0 = 0.0, acc1 = 0.0;
-                int tt = 0;
-                for (; tt + 2 <= j; tt += 2) {
-                    acc0 += up[tt] * ul[tt];
-                    acc1 += up[tt + 1] * ul[tt + 1];
+                double acc[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int tt = 0; tt < j; tt += 8) {     // zero terms past j: whole batches
+                    dv2 a[4], c[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        a[k] = *reinterpret_cast<const dv2*>(up + tt + 2 * k);
+                        c[k] = *reinterpret_cast<const dv2*>(ul + tt + 2 * k);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        acc[k] += a[k].x * c[k].x;
+                        acc[k] += a[k].y * c[k].y;
+                    }
                 }
-                if (tt < j) acc0 += up[tt] * ul[tt];
-                u = (L.V[p * R + l] - (acc0 + acc1)) * inv;
+                u = (L.V[p * R + l] - ((acc[0] + acc[1]) + (acc[2] + acc[3]))) * inv;
                 sumsq += u * u;
             }
             if (l < R) L.UbT[l * LDT + j] = u;

@@ -56,6 +56,28 @@ __device__ __forceinline__ void wave_argmax(double& v, int& i) {
     i = bi;
 }
 
+// Wave-wide argmax as max + ballot: the first lane holding the largest value
+// (NaN never wins; 64 when no lane compares equal, i.e. every lane is NaN).
+template <int CTRL>
+__device__ __forceinline__ double max_dpp_step(double v) {
+    const int olo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int ohi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return fmax(v, __hiloint2double(ohi, olo));
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+__device__ __forceinline__ int wave_first_max(double v, double& vmax) {
+    double r = max_dpp_step<0x128>(v);   // row_ror:8
+    r = max_dpp_step<0x124>(r);          // row_ror:4
+    r = max_dpp_step<0x122>(r);          // row_ror:2
+    r = max_dpp_step<0x121>(r);          // row_ror:1
+    vmax = fmax(fmax(lane_value(r, 0), lane_value(r, 16)), fmax(lane_value(r, 32), lane_value(r, 48)));
+    const uint64_t hit = __builtin_amdgcn_ballot_w64(v == vmax);
+    return hit ? (int)__builtin_ctzll(hit) : 64;
+}
+
 // ----------------------------------------------------- batched global loads
 // Small kernels here are latency bound: every helper issues all of its loads
 // before the first use, so a thread waits on memory once per batch.

@@ -16,6 +16,7 @@
 namespace itcpd {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d16 __attribute__((ext_vector_type(16)));
 
 // v_mfma_f64_16x16x4f64 on gfx950 (layout measured on the device):
 //   A operand: lane l holds A[l & 15][l >> 4]
@@ -92,6 +93,8 @@ struct itcpd_plan {
     double* U = nullptr;                       // R x R factor workspace (Cholesky / QR)
     double* qr_aux = nullptr;                  // QR fallback workspace (tau, Z reflectors)
     double* invdiag = nullptr;                 // R: 1 / U(j,j) of the pivoted Cholesky
+    double* Upad = nullptr;                    // R <= 64: V^-1 (R x R) from the factor kernel
+    int RT = 0;                                // 16, 32 or 64 for R <= 64, else 0
     double* Msum = nullptr;                    // maxI x R: summed MTTKRP of the last mode (fit)
     unsigned* counters = nullptr;              // arrival counters of last-block reductions
     int* piv = nullptr;                        // R

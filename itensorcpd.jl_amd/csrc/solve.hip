@@ -110,17 +110,21 @@ __global__ __launch_bounds__(FACTOR_THREADS) void k_factor(GramSet gs, int mode,
     qr_fallback<NT>(gs, mode, R, W, qtau, ztau, jpvt, vwork, finfo, redv, redi, rank);
 }
 
-// Pivoted Cholesky for R <= 64 in one wave, no barriers in the pivot loop.
-// dpstf2 left-looking form on the unpermuted matrix V held in LDS: at step j the
-// pivot p is the remaining index with the largest V(p,p) - sum_t U(t,p)^2 (first
-// index on ties); row j of U is u_l = (V(p,l) - sum_{t<j} U(t,p) U(t,l)) / U(j,j)
-// for the remaining l.  Output in pivoted order: Uperm(j, t) = u^(j)_{piv[t]},
-// invdiag(j) = 1/U(j,j).  On rank deficiency all four waves build the QR data.
-// U is kept transposed, UbT(l, t) = U(t, l) with ld 66, so the
-// dot product sum_{t<j} U(t,p) U(t,l) reads row p (uniform: broadcast) and row l
-// of UbT with 16-byte loads at constant offsets, 16 terms per batch.
+// Pivoted Cholesky for R <= 64 (dpstrf semantics as k_factor: absolute tolerance,
+// the first step only checks d > 0): pivot p is the remaining index with the
+// largest V(p,p) - sum_t U(t,p)^2 (first index on ties).  Right-looking on
+// registers: thread (w, l) holds rows 16w..16w+15 of column l of the Schur
+// complement; each step the wave owning row p hands it to all waves through LDS,
+// u_l = S(p, l) / U(j,j) for the remaining l, and every thread applies the rank-1
+// update to its 16 entries.  The pivot search runs redundantly in every wave on
+// identical data (the same u, the same running sums), so all waves agree without
+// communication.  UbT(l, j) = U(j, l) (zero for finished l) with ld 66.
+// Output in pivoted order: Uperm(j, t) = u^(j)_{piv[t]}, invdiag(j) = 1/U(j,j);
+// with Vinv, also V^-1 (R x R).  On rank deficiency all four waves build the QR
+// data.
 __global__ __launch_bounds__(256) void k_factor64(GramSet gs, int mode, int R, double tol,
                                                   double* __restrict__ Uperm,
+                                                  double* __restrict__ Vinv,
                                                   double* __restrict__ invdiag,
                                                   int* __restrict__ piv, int* __restrict__ finfo,
                                                   double* __restrict__ W, double* __restrict__ qtau,
@@ -128,64 +132,58 @@ __global__ __launch_bounds__(256) void k_factor64(GramSet gs, int mode, int R, d
                                                   double* vwork, const DevCheck* __restrict__ chk) {
     if (chk && chk->done) return;
     constexpr int LDT = 66;
-    extern __shared__ double S[];   // V: R x R | UbT: 64 x LDT
+    extern __shared__ double S[];   // V (ld R), later U (ld 64) | UbT: 64 x LDT
     __shared__ double diagU[64];
+    __shared__ double invdU[64];
+    __shared__ double rowb[2][64];
     __shared__ int pivS[64];
     __shared__ int s_rank;
     __shared__ double redv[256];
     __shared__ int redi[256];
-    double* UbT = S + ((R * R + 1) & ~1);
-    const int t = threadIdx.x;
+    double* UbT = S + 64 * 64;
+    const int t = threadIdx.x, l = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     fill_hadamard<256>(S, gs, mode, R * R);
     __syncthreads();
-    if (t < 64) {
-        const int l = t;
-        bool rem = l < R;
-        const double vdiag = rem ? S[l * R + l] : 0.0;
-        double sumsq = 0.0;
-        int rank = R;
-        for (int j = 0; j < R; ++j) {
-            double best = rem ? vdiag - sumsq : -__builtin_inf();
-            int bi = rem ? l : R;
-            wave_argmax(best, bi);
-            const int p = __builtin_amdgcn_readfirstlane(bi);
-            const double d = best;
-            const bool bad = (p >= R) || ((j == 0) ? !(d > 0.0) : !(d > tol));
-            if (bad) { rank = j; break; }
-            const double ujj = sqrt(d);
-            const double inv = 1.0 / ujj;
-            if (l == 0) { diagU[j] = ujj; pivS[j] = p; }
-            if (l == p) rem = false;
-            double u = 0.0;
-            if (rem) {
-                const double* __restrict__ up = UbT + p * LDT;
-                const double* __restrict__ ul = UbT + l * LDT;
-                double acc[4] = {0.0, 0.0, 0.0, 0.0};
-                int tt = 0;
-                for (; tt + 16 <= j; tt += 16) {
-                    dv2 a[8], b[8];
+    d16 sr;
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        a[k] = *reinterpret_cast<const dv2*>(up + tt + 2 * k);
-                        b[k] = *reinterpret_cast<const dv2*>(ul + tt + 2 * k);
-                    }
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        acc[k & 3] += a[k].x * b[k].x;
-                        acc[k & 3] += a[k].y * b[k].y;
-                    }
-                }
-                for (; tt < j; ++tt) acc[0] += up[tt] * ul[tt];
-                u = (S[p * R + l] - ((acc[0] + acc[1]) + (acc[2] + acc[3]))) * inv;
-                sumsq += u * u;
-            }
-            if (l < R) UbT[l * LDT + j] = u;   // zero for finished indices
-        }
-        if (l == 0) s_rank = rank;
+    for (int i = 0; i < 16; ++i) {
+        const int m = 16 * w + i;
+        sr[i] = (m < R && l < R) ? S[m * R + l] : 0.0;
     }
+    bool rem = l < R;
+    const double vdiag = rem ? S[l * R + l] : 0.0;
+    double sumsq = 0.0;
+    uint64_t remmask = R >= 64 ? ~0ull : ((1ull << R) - 1);
+    int rank = R;
+    for (int j = 0; j < R; ++j) {
+        double d;
+        const int p = wave_first_max(rem ? vdiag - sumsq : -__builtin_inf(), d);
+        const bool bad = (p >= R) || !((remmask >> p) & 1) || ((j == 0) ? !(d > 0.0) : !(d > tol));
+        if (bad) { rank = j; break; }
+        if (w == (p >> 4)) rowb[j & 1][l] = sr[p & 15];
+        const double ujj = sqrt(d);
+        const double inv = 1.0 / ujj;
+        if (t == 0) { diagU[j] = ujj; invdU[j] = inv; pivS[j] = p; }
+        remmask &= ~(1ull << p);
+        if (l == p) rem = false;
+        __syncthreads();
+        // rows of finished indices take the update too (never read again), so the
+        // 16 row values are plain loads with no per-row condition
+        const double* rb = rowb[j & 1];
+        const double rl = rb[l];
+        double rw[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) rw[i] = rb[16 * w + i];
+        const double u = rem ? rl * inv : 0.0;
+        sumsq += u * u;
+        if (w == 0 && l < R) UbT[l * LDT + j] = u;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sr[i] -= (rw[i] * inv) * u;
+    }
+    if (t == 0) s_rank = rank;
     __syncthreads();
-    const int rank = s_rank;
-    if (rank == R) {
+    if (s_rank == R) {
         for (int x = t; x < R * R; x += 256) {
             const int j = x / R, tt = x % R;
             double v = 0.0;
@@ -194,13 +192,96 @@ __global__ __launch_bounds__(256) void k_factor64(GramSet gs, int mode, int R, d
             Uperm[x] = v;
         }
         for (int j = t; j < R; j += 256) {
-            invdiag[j] = 1.0 / diagU[j];
+            invdiag[j] = invdU[j];
             piv[j] = pivS[j];
+        }
+        // V^-1 = P U^-1 U^-T P^T for the MFMA row solves (Vinv != null):
+        // U (pivoted order, identity beyond R) into S with ld 64; U^-1 into UbT by
+        // 16 x 16 blocks: diagonal blocks by substitution (wave b, lane c: column c
+        // of block b), then X_ij = -X_ii sum_{k=i+1..j} U_ik X_kj by block
+        // diagonals on MFMA; finally Z = U^-1 U^-T on MFMA and Vinv(piv a, piv b) =
+        // Z(a, b).  f64 16x16x4 MFMA layout: A lane (c, g) = A[c][k0+g], B lane =
+        // B[k0+g][c], result v = D[g+4v][c]; the result of one product is the B
+        // operand of the next one as it stands (k0 = 4v).
+        if (Vinv) {
+            const int c = l & 15, g = l >> 4;
+            for (int x = t; x < 64 * 64; x += 256) {
+                const int i = x >> 6, k = x & 63;
+                double v = 0.0;
+                if (i < R && k < R) {
+                    if (k > i) v = UbT[pivS[k] * LDT + i];
+                    else if (k == i) v = diagU[i];
+                } else if (i == k) {
+                    v = 1.0;
+                }
+                S[x] = v;
+            }
+            if (t >= R && t < 64) invdU[t] = 1.0;
+            __syncthreads();
+            for (int x = t; x < 64 * 64; x += 256) {
+                const int i = x >> 6, k = x & 63;
+                if ((i >> 4) != (k >> 4)) UbT[i * LDT + k] = 0.0;
+            }
+            if (l < 16) {
+                const int b0 = 16 * w;
+                double xv[16];
+#pragma unroll
+                for (int i = 15; i >= 0; --i) {
+                    double acc0 = (i == l) ? 1.0 : 0.0, acc1 = 0.0;
+#pragma unroll
+                    for (int k = i + 1; k < 16; ++k) {
+                        if (k & 1) acc1 -= S[(b0 + i) * 64 + b0 + k] * xv[k];
+                        else acc0 -= S[(b0 + i) * 64 + b0 + k] * xv[k];
+                    }
+                    xv[i] = (i <= l) ? (acc0 + acc1) * invdU[b0 + i] : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < 16; ++i) UbT[(b0 + i) * LDT + b0 + l] = xv[i];
+            }
+            __syncthreads();
+            for (int dd = 1; dd < 4; ++dd) {
+                const int bi = w, bj = w + dd;
+                if (bj < 4) {
+                    d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+                    for (int bk = bi + 1; bk <= bj; ++bk)
+#pragma unroll
+                        for (int k0 = 0; k0 < 16; k0 += 4)
+                            acc = mfma_f64(S[(16 * bi + c) * 64 + 16 * bk + k0 + g],
+                                           UbT[(16 * bk + k0 + g) * LDT + 16 * bj + c], acc);
+                    d4 out = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        out = mfma_f64(-UbT[(16 * bi + c) * LDT + 16 * bi + 4 * v + g], acc[v], out);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) UbT[(16 * bi + g + 4 * v) * LDT + 16 * bj + c] = out[v];
+                }
+                __syncthreads();
+            }
+            // Z(a, b) = sum_k Uinv(a, k) Uinv(b, k); wave w does output row tile w
+            if (w * 16 < R) {
+                d4 acc[4];
+#pragma unroll
+                for (int bt = 0; bt < 4; ++bt) acc[bt] = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k0 = 0; k0 < 64; k0 += 4) {
+                    const double av = UbT[(16 * w + c) * LDT + k0 + g];
+#pragma unroll
+                    for (int bt = 0; bt < 4; ++bt)
+                        acc[bt] = mfma_f64(av, UbT[(16 * bt + c) * LDT + k0 + g], acc[bt]);
+                }
+#pragma unroll
+                for (int bt = 0; bt < 4; ++bt)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int ra = 16 * w + g + 4 * v, rb = 16 * bt + c;
+                        if (ra < R && rb < R) Vinv[pivS[ra] * R + pivS[rb]] = acc[bt][v];
+                    }
+            }
         }
         if (t == 0) { finfo[0] = R; finfo[1] = 0; finfo[2] = R; }
         return;
     }
-    qr_fallback<256>(gs, mode, R, W, qtau, ztau, jpvt, vwork, finfo, redv, redi, rank);
+    qr_fallback<256>(gs, mode, R, W, qtau, ztau, jpvt, vwork, finfo, redv, redi, s_rank);
 }
 
 // Row solves and Gram tiles (no cross-block communication inside the kernel):
@@ -217,8 +298,9 @@ __global__ __launch_bounds__(256) void k_solve(
     const int* __restrict__ finfo, const double* __restrict__ W, const double* __restrict__ qtau,
     const double* __restrict__ ztau, const int* __restrict__ jpvt, int u_in_lds,
     double* __restrict__ X, double* __restrict__ Msum, double* __restrict__ gram_part, int grams,
-    const DevCheck* __restrict__ chk) {
+    int only_fallback, const DevCheck* __restrict__ chk) {
     if (chk->done) return;
+    if (only_fallback && finfo[1] == 0) return;   // the Cholesky case went to k_solve_mfma
     constexpr int RPB = 4 * RPW;
     extern __shared__ double lds[];
     const int t = threadIdx.x;
@@ -343,6 +425,85 @@ __global__ __launch_bounds__(256) void k_solve(
                 double sacc = 0.0;
                 for (int ii = 0; ii < nr; ++ii) sacc += Xs[ii * R + r] * Xs[ii * R + r2];
                 gp[x] = first_tile ? sacc : gp[x] + sacc;
+            }
+        }
+        __syncthreads();
+        first_tile = false;
+    }
+}
+
+// Row solves for R <= RT <= 64 with the explicit V^-1 the factor kernel formed:
+// X = M V^-1 on FP64 MFMA.  A block takes 64-row tiles, wave w rows 16w..16w+15,
+// k in steps of 4 (f64 16x16x4 layout: A lane l = A[l&15][l>>4], B lane l =
+// B[l>>4][l&15], result v of lane l = D[(l>>4) + 4v][l&15]).  The tile of X goes
+// through LDS into the block's X^T X partial, also on MFMA.  M is copied to Msum
+// when asked.  Exits at once when the factor kernel took the QR fallback
+// (k_solve handles that case).
+template <int RT>
+__global__ __launch_bounds__(256) void k_solve_mfma(const double* __restrict__ Msrc, int64_t In, int R,
+                                                    const double* __restrict__ Vinv,
+                                                    const int* __restrict__ finfo,
+                                                    double* __restrict__ X, double* __restrict__ Msum,
+                                                    double* __restrict__ gram_part, int grams,
+                                                    const DevCheck* __restrict__ chk) {
+    if (chk->done || finfo[1] != 0) return;
+    constexpr int NT = RT / 16;
+    constexpr int LDX = RT + 2;
+    __shared__ double Xs[64 * LDX];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, c = l & 15, g = l >> 4;
+    double* gp = gram_part + (int64_t)blockIdx.x * R * R;
+    bool first_tile = true;
+    for (int64_t i0 = (int64_t)blockIdx.x * 64; i0 < In; i0 += (int64_t)gridDim.x * 64) {
+        d4 acc[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[nt] = (d4){0.0, 0.0, 0.0, 0.0};
+        const int64_t ia = i0 + 16 * w + c;
+        const bool aok = ia < In;
+#pragma unroll
+        for (int k0 = 0; k0 < RT; k0 += 4) {
+            const int k = k0 + g;
+            const bool kok = k < R;
+            const double av = (aok && kok) ? Msrc[ia + In * k] : 0.0;
+            if (Msum && aok && kok) Msum[ia + In * k] = av;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int col = 16 * nt + c;
+                const double bv = (kok && col < R) ? Vinv[k * R + col] : 0.0;
+                acc[nt] = mfma_f64(av, bv, acc[nt]);
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rl = 16 * w + g + 4 * v, col = 16 * nt + c;
+                const int64_t row = i0 + rl;
+                const bool ok = row < In && col < R;
+                if (ok) X[row + In * col] = acc[nt][v];
+                Xs[rl * LDX + col] = ok ? acc[nt][v] : 0.0;
+            }
+        __syncthreads();
+        if (grams) {
+            for (int ta = w; ta < NT; ta += 4) {
+                d4 gacc[NT];
+#pragma unroll
+                for (int tb = 0; tb < NT; ++tb) gacc[tb] = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < 64; kk += 4) {
+                    const double av = Xs[(kk + g) * LDX + 16 * ta + c];
+#pragma unroll
+                    for (int tb = 0; tb < NT; ++tb) gacc[tb] = mfma_f64(av, Xs[(kk + g) * LDX + 16 * tb + c], gacc[tb]);
+                }
+#pragma unroll
+                for (int tb = 0; tb < NT; ++tb)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = 16 * ta + g + 4 * v, r2 = 16 * tb + c;
+                        if (r < R && r2 < R) {
+                            double* q = gp + r * R + r2;
+                            *q = first_tile ? gacc[tb][v] : *q + gacc[tb][v];
+                        }
+                    }
             }
         }
         __syncthreads();
@@ -736,9 +897,9 @@ int launch_factor_on(itcpd_plan* p, int mode, hipStream_t st) {
     double* ztau = qtau + R;
     double* vwork = ztau + R;
     if (R <= 64) {
-        hipLaunchKernelGGL(k_factor64, dim3(1), dim3(256), (size_t)(((R * R + 1) & ~1) + 64 * 66) * 8,
-                           st, grams_of(p), mode, R, 1e-6, p->U, p->invdiag, p->piv, p->finfo, W,
-                           qtau, ztau, p->piv + R, vwork, p->chk);
+        hipLaunchKernelGGL(k_factor64, dim3(1), dim3(256), (size_t)(64 * 64 + 64 * 66) * 8,
+                           st, grams_of(p), mode, R, 1e-6, p->U, p->Upad, p->invdiag, p->piv,
+                           p->finfo, W, qtau, ztau, p->piv + R, vwork, p->chk);
     } else {
         // right-looking, all threads; V in LDS up to R = 128, else in the U buffer
         const bool lds = R <= 128;
@@ -762,7 +923,8 @@ int rpb_of(int R) { return 4 * (R <= 256 ? 4 : (R <= 512 ? 2 : 1)); }
 int solve_nb(int64_t In, int R) {
     int64_t cap = ((int64_t)1 << 25) / ((int64_t)R * R);
     cap = cap < 1 ? 1 : (cap > 256 ? 256 : cap);
-    const int64_t nb = cdiv(In, rpb_of(R));
+    // R <= 64: 64-row blocks (k_solve_mfma; k_solve's 16-row tiles then reach every block)
+    const int64_t nb = cdiv(In, R <= 64 ? 64 : rpb_of(R));
     return (int)(nb < cap ? nb : cap);
 }
 int finalize_nb(int64_t In) {
@@ -782,7 +944,27 @@ int run_solve_k(itcpd_plan* p, int mode, const double* Msrc, int nparts, int64_t
     double* W = p->qr_aux;
     hipLaunchKernelGGL((k_solve<RQ, RPW>), dim3(nb), dim3(256), lds, p->stream, Msrc, In, R, p->U, p->invdiag, p->piv, p->finfo, W, W + (size_t)R * R,
                        W + (size_t)R * R + R, (const int*)(p->piv + R), u_lds ? 1 : 0, p->X,
-                       keep_m ? p->Msum : (double*)nullptr, p->gram_part, grams ? 1 : 0, p->chk);
+                       keep_m ? p->Msum : (double*)nullptr, p->gram_part, grams ? 1 : 0, 0, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+// R <= 64: MFMA product with V^-1 for the Cholesky case, k_solve for the QR fallback
+// (each exits at once in the other case); both use the same nb row blocks.
+template <int RT>
+int run_solve_rows(itcpd_plan* p, int mode, const double* Msrc, bool grams, bool keep_m) {
+    const int64_t In = p->dims[mode];
+    const int R = (int)p->R;
+    const int nb = solve_nb(In, R);
+    double* Msum = keep_m ? p->Msum : (double*)nullptr;
+    hipLaunchKernelGGL((k_solve_mfma<RT>), dim3(nb), dim3(256), 0, p->stream, Msrc, In, R, p->Upad, p->finfo,
+                       p->X, Msum, p->gram_part, grams ? 1 : 0, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    double* W = p->qr_aux;
+    const size_t lds = (size_t)16 * R * 8 + (size_t)R * (R + 1) * 8;
+    hipLaunchKernelGGL((k_solve<1, 4>), dim3(nb), dim3(256), lds, p->stream, Msrc, In, R, p->U, p->invdiag,
+                       p->piv, p->finfo, W, W + (size_t)R * R, W + (size_t)R * R + R,
+                       (const int*)(p->piv + R), 1, p->X, Msum, p->gram_part, grams ? 1 : 0, 1, p->chk);
     ITCPD_HIP(hipGetLastError());
     return ITCPD_OK;
 }
@@ -790,6 +972,9 @@ int run_solve_k(itcpd_plan* p, int mode, const double* Msrc, int nparts, int64_t
 int run_apply(itcpd_plan* p, int mode, const double* Msrc, int nparts, int64_t stride, bool grams,
               bool keep_m) {
     const int R = (int)p->R;
+    if (p->RT == 16) return run_solve_rows<16>(p, mode, Msrc, grams, keep_m);
+    if (p->RT == 32) return run_solve_rows<32>(p, mode, Msrc, grams, keep_m);
+    if (p->RT == 64) return run_solve_rows<64>(p, mode, Msrc, grams, keep_m);
     if (R <= 64) return run_solve_k<1>(p, mode, Msrc, nparts, stride, grams, keep_m);
     if (R <= 128) return run_solve_k<2>(p, mode, Msrc, nparts, stride, grams, keep_m);
     if (R <= 256) return run_solve_k<4>(p, mode, Msrc, nparts, stride, grams, keep_m);

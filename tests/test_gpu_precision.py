@@ -75,11 +75,13 @@ def test_collinear_als_fit(itcpd, prec, fit_bar, dims, R):
         plan.run(cg)
         got = plan.get_cpd()
     # the fit of the returned CPD evaluated in FP64 (explicit residual) is within the
-    # bar of the FP64 run's fit; so is the fit the reduced-precision run reported (its
-    # <T, X> comes from the last mode's reduced-precision MTTKRP)
+    # bar of the FP64 run's fit.  The fit the reduced-precision run reports takes
+    # <T, X> from the last mode's reduced-precision MTTKRP: a relative error e of
+    # <T, X> moves the fit by about e / (1 - fit), so its bar adds that term for
+    # e = 2^-22 (two FP32 roundings)
     fit_true = 1 - np.linalg.norm(O.reconstruct(got.factors, got.lam) - T64) / nrm
     assert abs(fit_true - cc.final_fit) < fit_bar
-    assert abs(cg.final_fit - cc.final_fit) < fit_bar
+    assert abs(cg.final_fit - cc.final_fit) < fit_bar + 2.0 ** -22 / (1 - cc.final_fit)
     assert max(rel(a, b) for a, b in zip(got.factors, ref.factors)) < 3e-2
 
 
