@@ -162,8 +162,20 @@ __global__ __launch_bounds__(256) void k_factor64(GramSet gs, int mode, int R, d
         const bool bad = (p >= R) || !((remmask >> p) & 1) || ((j == 0) ? !(d > 0.0) : !(d > tol));
         if (bad) { rank = j; break; }
         if (w == (p >> 4)) rowb[j & 1][l] = sr[p & 15];
-        const double ujj = sqrt(d);
-        const double inv = 1.0 / ujj;
+        // 1/sqrt(d) from the hardware estimate (relative error < 2^-22) and two
+        // Newton steps y <- y + y (1/2 - d y^2 / 2); U(j,j) = d / sqrt(d)
+        double inv, ujj;
+        if (d > 0x1p-900) {
+            const double h = 0.5 * d;
+            double y = __builtin_amdgcn_rsq(d);
+            y = fma(y, fma(-h * y, y, 0.5), y);
+            y = fma(y, fma(-h * y, y, 0.5), y);
+            inv = y;
+            ujj = d * y;
+        } else {
+            ujj = sqrt(d);
+            inv = 1.0 / ujj;
+        }
         if (t == 0) { diagU[j] = ujj; invdU[j] = inv; pivS[j] = p; }
         remmask &= ~(1ull << p);
         if (l == p) rem = false;
@@ -433,81 +445,84 @@ __global__ __launch_bounds__(256) void k_solve(
 }
 
 // Row solves for R <= RT <= 64 with the explicit V^-1 the factor kernel formed:
-// X = M V^-1 on FP64 MFMA.  A block takes 64-row tiles, wave w rows 16w..16w+15,
-// k in steps of 4 (f64 16x16x4 layout: A lane l = A[l&15][l>>4], B lane l =
-// B[l>>4][l&15], result v of lane l = D[(l>>4) + 4v][l&15]).  The tile of X goes
-// through LDS into the block's X^T X partial, also on MFMA.  M is copied to Msum
-// when asked.  Exits at once when the factor kernel took the QR fallback
-// (k_solve handles that case).
+// X = M V^-1 on FP64 MFMA.  A block of RT/16 waves takes 16-row tiles (grid
+// stride); wave w owns output columns 16w..16w+15 and keeps its V^-1 column panel
+// in registers for all tiles.  f64 16x16x4 layout: A lane (c, g) = A[c][k0+g],
+// B lane = B[k0+g][c], result v = D[g+4v][c].  The wave's result tile is, as it
+// stands, the A operand of X^T X over the tile's rows (k0 = 4v), so the block's
+// Gram partial needs only the other waves' tiles from LDS; it stays in registers
+// across tiles and is written once.  All loads of a tile are issued before the
+// first MFMA.  M is copied to Msum when asked.  Exits at once when the factor
+// kernel took the QR fallback (k_solve handles that case).
 template <int RT>
-__global__ __launch_bounds__(256) void k_solve_mfma(const double* __restrict__ Msrc, int64_t In, int R,
-                                                    const double* __restrict__ Vinv,
-                                                    const int* __restrict__ finfo,
-                                                    double* __restrict__ X, double* __restrict__ Msum,
-                                                    double* __restrict__ gram_part, int grams,
-                                                    const DevCheck* __restrict__ chk) {
+__global__ __launch_bounds__(RT * 4) void k_solve_mfma(const double* __restrict__ Msrc, int64_t In, int R,
+                                                       const double* __restrict__ Vinv,
+                                                       const int* __restrict__ finfo,
+                                                       double* __restrict__ X, double* __restrict__ Msum,
+                                                       double* __restrict__ gram_part, int grams,
+                                                       const DevCheck* __restrict__ chk) {
     if (chk->done || finfo[1] != 0) return;
-    constexpr int NT = RT / 16;
-    constexpr int LDX = RT + 2;
-    __shared__ double Xs[64 * LDX];
-    const int t = threadIdx.x, w = t >> 6, l = t & 63, c = l & 15, g = l >> 4;
-    double* gp = gram_part + (int64_t)blockIdx.x * R * R;
-    bool first_tile = true;
-    for (int64_t i0 = (int64_t)blockIdx.x * 64; i0 < In; i0 += (int64_t)gridDim.x * 64) {
-        d4 acc[NT];
+    constexpr int NT = RT / 16, KS = RT / 4, LDX = RT + 2;
+    __shared__ double Xs[16 * LDX];
+    const int t = threadIdx.x, l = t & 63, c = l & 15, g = l >> 4;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int col = 16 * w + c;
+    const bool colok = col < R;
+    double bv[KS];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[nt] = (d4){0.0, 0.0, 0.0, 0.0};
-        const int64_t ia = i0 + 16 * w + c;
+    for (int ks = 0; ks < KS; ++ks) {
+        const int k = 4 * ks + g;
+        bv[ks] = (colok && k < R) ? Vinv[k * R + col] : 0.0;
+    }
+    d4 gacc[NT];
+#pragma unroll
+    for (int tb = 0; tb < NT; ++tb) gacc[tb] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int64_t i0 = (int64_t)blockIdx.x * 16; i0 < In; i0 += (int64_t)gridDim.x * 16) {
+        const int64_t ia = i0 + c;
         const bool aok = ia < In;
+        double av[KS];
 #pragma unroll
-        for (int k0 = 0; k0 < RT; k0 += 4) {
-            const int k = k0 + g;
-            const bool kok = k < R;
-            const double av = (aok && kok) ? Msrc[ia + In * k] : 0.0;
-            if (Msum && aok && kok) Msum[ia + In * k] = av;
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = 4 * ks + g;
+            av[ks] = (aok && k < R) ? Msrc[ia + In * k] : 0.0;
+        }
+        if (Msum) {
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const int col = 16 * nt + c;
-                const double bv = (kok && col < R) ? Vinv[k * R + col] : 0.0;
-                acc[nt] = mfma_f64(av, bv, acc[nt]);
+            for (int ks = 4 * w; ks < 4 * w + 4; ++ks) {
+                const int k = 4 * ks + g;
+                if (aok && k < R) Msum[ia + In * k] = av[ks];
             }
         }
+        d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
+        for (int ks = 0; ks < KS; ++ks) acc = mfma_f64(av[ks], bv[ks], acc);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = i0 + g + 4 * v;
+            if (row < In && colok) X[row + In * col] = acc[v];
+        }
+        if (grams) {
+            // rows >= In and columns >= R of acc are zero (their operands were)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Xs[(g + 4 * v) * LDX + col] = acc[v];
+            __syncthreads();
+#pragma unroll
+            for (int tb = 0; tb < NT; ++tb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    gacc[tb] = mfma_f64(acc[v], Xs[(4 * v + g) * LDX + 16 * tb + c], gacc[tb]);
+            __syncthreads();
+        }
+    }
+    if (grams) {
+        double* gp = gram_part + (int64_t)blockIdx.x * R * R;
+#pragma unroll
+        for (int tb = 0; tb < NT; ++tb)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int rl = 16 * w + g + 4 * v, col = 16 * nt + c;
-                const int64_t row = i0 + rl;
-                const bool ok = row < In && col < R;
-                if (ok) X[row + In * col] = acc[nt][v];
-                Xs[rl * LDX + col] = ok ? acc[nt][v] : 0.0;
+                const int r = 16 * w + g + 4 * v, r2 = 16 * tb + c;
+                if (r < R && r2 < R) gp[r * R + r2] = gacc[tb][v];
             }
-        __syncthreads();
-        if (grams) {
-            for (int ta = w; ta < NT; ta += 4) {
-                d4 gacc[NT];
-#pragma unroll
-                for (int tb = 0; tb < NT; ++tb) gacc[tb] = (d4){0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int kk = 0; kk < 64; kk += 4) {
-                    const double av = Xs[(kk + g) * LDX + 16 * ta + c];
-#pragma unroll
-                    for (int tb = 0; tb < NT; ++tb) gacc[tb] = mfma_f64(av, Xs[(kk + g) * LDX + 16 * tb + c], gacc[tb]);
-                }
-#pragma unroll
-                for (int tb = 0; tb < NT; ++tb)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int r = 16 * ta + g + 4 * v, r2 = 16 * tb + c;
-                        if (r < R && r2 < R) {
-                            double* q = gp + r * R + r2;
-                            *q = first_tile ? gacc[tb][v] : *q + gacc[tb][v];
-                        }
-                    }
-            }
-        }
-        __syncthreads();
-        first_tile = false;
     }
 }
 
@@ -923,8 +938,8 @@ int rpb_of(int R) { return 4 * (R <= 256 ? 4 : (R <= 512 ? 2 : 1)); }
 int solve_nb(int64_t In, int R) {
     int64_t cap = ((int64_t)1 << 25) / ((int64_t)R * R);
     cap = cap < 1 ? 1 : (cap > 256 ? 256 : cap);
-    // R <= 64: 64-row blocks (k_solve_mfma; k_solve's 16-row tiles then reach every block)
-    const int64_t nb = cdiv(In, R <= 64 ? 64 : rpb_of(R));
+    // R <= 64: 16-row tiles, both in k_solve_mfma and in k_solve's fallback path
+    const int64_t nb = cdiv(In, R <= 64 ? 16 : rpb_of(R));
     return (int)(nb < cap ? nb : cap);
 }
 int finalize_nb(int64_t In) {
@@ -957,7 +972,7 @@ int run_solve_rows(itcpd_plan* p, int mode, const double* Msrc, bool grams, bool
     const int R = (int)p->R;
     const int nb = solve_nb(In, R);
     double* Msum = keep_m ? p->Msum : (double*)nullptr;
-    hipLaunchKernelGGL((k_solve_mfma<RT>), dim3(nb), dim3(256), 0, p->stream, Msrc, In, R, p->Upad, p->finfo,
+    hipLaunchKernelGGL((k_solve_mfma<RT>), dim3(nb), dim3(RT * 4), 0, p->stream, Msrc, In, R, p->Upad, p->finfo,
                        p->X, Msum, p->gram_part, grams ? 1 : 0, p->chk);
     ITCPD_HIP(hipGetLastError());
     double* W = p->qr_aux;
