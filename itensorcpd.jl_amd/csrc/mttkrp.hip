@@ -158,6 +158,129 @@ __global__ __launch_bounds__(256, 2) void k_tk(const TT* __restrict__ T, int64_t
         }
 }
 
+// ------------------------------------------------- TK GEMM, LDS ring form
+// The same product for FP64 T, 64-rank blocks and 256-row workgroup tiles, with
+// T and the KRP rows staged through a ring of LDS slots filled by direct
+// global->LDS loads (global_load_lds_dwordx4: lane l of a wave writes 16 bytes at
+// slot offset + 16 l).  A stage is U = 2 four-steps, i.e. 8 consecutive
+// reduction indices i of one rest: 8 T columns of 256 rows (2 KB each, padded to
+// TKR_CST bytes) and 8 rows of At0 (64 ranks, 512 bytes each).  Stage q+2 is
+// requested while stage q is computed; the loads bypass registers, so the
+// bytes in flight no longer compete with the accumulators.
+//
+// Synchronisation per stage: each wave waits for its own loads of the stage
+// (vmcnt: loads return in order, so at most the 5 loads of stage q+1 may still
+// be pending) and for its LDS reads (lgkmcnt 0), then the workgroup barrier makes
+// every wave's loads visible; the slot refilled after that barrier is the one
+// read in the previous stage, which every wave has finished.
+constexpr int TKR_NS = 3;
+constexpr int TKR_CST = 2048 + 64;
+constexpr int TKR_TBYTES = 8 * TKR_CST;
+constexpr int TKR_SLOT = TKR_TBYTES + 8 * 512;
+constexpr int TKR_LDS = TKR_NS * TKR_SLOT;
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+__device__ __forceinline__ void lds_dma16(const void* src, unsigned char* dst) {
+    __builtin_amdgcn_global_load_lds(src, (lds_void_t*)dst, 16, 0, 0);
+}
+// s_waitcnt: vmcnt <= n (n < 16), lgkmcnt = 0, expcnt not waited for
+#define TKR_WAIT(n) __builtin_amdgcn_s_waitcnt(0x0000 | ((n) & 0xf) | (0x7 << 4) | (0x0 << 8))
+
+__global__ __launch_bounds__(256, 2) void k_tk_ring(const double* __restrict__ T, int64_t ML, Side sd,
+                                                    int64_t nsteps_i, int64_t kk_per_split,
+                                                    int64_t kk_total, int Rp, int R,
+                                                    double* __restrict__ out, int64_t out_ld,
+                                                    int64_t split_stride,
+                                                    const DevCheck* __restrict__ chk) {
+    constexpr int RB = 4, MB = 4, U = 2;
+    if (chk->done) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char ring[];
+    const int t = threadIdx.x, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int c = lane & 15, g = lane >> 4;
+    const int64_t mwg = (int64_t)blockIdx.x * 256;
+    const int r0 = blockIdx.y * 64;
+    const int64_t kk0 = (int64_t)blockIdx.z * kk_per_split;
+    const int64_t kk1 = kk0 + kk_per_split < kk_total ? kk0 + kk_per_split : kk_total;
+    const int nst = (int)((kk1 - kk0) / U);
+    const int64_t I0 = sd.dim[0];
+    const double* __restrict__ At0 = sd.fac[0];
+
+    // requests stage q into slot q % TKR_NS: wave w loads T columns 2w, 2w+1 (two
+    // 1 KB halves each) and At0 rows 2w, 2w+1 of the stage
+    auto issue = [&](int q) {
+        const int64_t kk = kk0 + (int64_t)q * U;
+        const int64_t rest = kk / nsteps_i;
+        const int64_t i0 = (kk - rest * nsteps_i) * 4;
+        unsigned char* slot = ring + (q % TKR_NS) * TKR_SLOT;
+#pragma unroll
+        for (int e2 = 0; e2 < 2; ++e2) {
+            const int e = 2 * wave + e2;
+            const double* col = T + (I0 * rest + i0 + e) * ML + mwg + 2 * lane;
+            lds_dma16(col, slot + e * TKR_CST);
+            lds_dma16(col + 128, slot + e * TKR_CST + 1024);
+        }
+        const int row = 2 * wave + (lane >> 5);
+        lds_dma16(At0 + (i0 + row) * Rp + r0 + 2 * (lane & 31), slot + TKR_TBYTES + 2 * wave * 512);
+    };
+
+    d4 acc[RB][MB];
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int b = 0; b < MB; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+
+    int64_t cur_rest = -1;
+    double w[RB];
+#pragma unroll
+    for (int a = 0; a < RB; ++a) w[a] = 0.0;
+    for (int q = 0; q < TKR_NS - 1 && q < nst; ++q) issue(q);
+    for (int q = 0; q < nst; ++q) {
+        if (q + 1 < nst) TKR_WAIT(5);
+        else TKR_WAIT(0);
+        __asm__ volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        __asm__ volatile("" ::: "memory");
+        if (q + TKR_NS - 1 < nst) issue(q + TKR_NS - 1);
+        const int64_t rest = (kk0 + (int64_t)q * U) / nsteps_i;
+        if (rest != cur_rest) {
+            cur_rest = rest;
+#pragma unroll
+            for (int a = 0; a < RB; ++a) w[a] = rest_weight(sd, rest, r0 + 16 * a + c, Rp, true);
+        }
+        const unsigned char* slot = ring + (q % TKR_NS) * TKR_SLOT;
+        double ta[U][MB], ka[U][RB];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const dv2* tp = reinterpret_cast<const dv2*>(slot + (4 * u + g) * TKR_CST + (wave * 64 + 4 * c) * 8);
+            const dv2 x0 = tp[0], x1 = tp[1];
+            ta[u][0] = x0.x; ta[u][1] = x0.y; ta[u][2] = x1.x; ta[u][3] = x1.y;
+            const double* kp = reinterpret_cast<const double*>(slot + TKR_TBYTES) + (4 * u + g) * 64 + c;
+#pragma unroll
+            for (int a = 0; a < RB; ++a) ka[u][a] = kp[16 * a] * w[a];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int a = 0; a < RB; ++a)
+#pragma unroll
+                for (int b = 0; b < MB; ++b) acc[a][b] = mfma_f64(ka[u][a], ta[u][b], acc[a][b]);
+    }
+
+    double* o = out + (int64_t)blockIdx.z * split_stride;
+    const int64_t mrow = mwg + wave * 64 + MB * c;
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + 16 * a + g + 4 * v;
+            if (r >= R) continue;
+#pragma unroll
+            for (int b = 0; b < MB; ++b) o[mrow + b + out_ld * r] = acc[a][b][v];
+        }
+}
+
 // ---------------------------------------------------------------- TR GEMM
 // out(mR, r) = sum over mL = i + I0*rest of T(mL, mR) KRP(mL, r),
 // KRP(mL, r) = A0[i + I0*r] * prod_{q>=1} Aq[digit_q + Iq*r] (column-major factors).
@@ -908,10 +1031,24 @@ int64_t pick_split(int64_t base_wgs, int64_t kk_total, int64_t target, int64_t m
     return nz;
 }
 
+// The LDS ring form of the left GEMM applies: FP64 T, 256-row tiles, 64-rank
+// blocks, stages of 8 indices of the first right mode.
+bool tk_ring_ok(const itcpd_plan* p, const Side& sd) {
+    return p->elt_bytes == 8 && p->RB == 4 && p->ML % 256 == 0 && p->Rp % 64 == 0 && sd.dim[0] % 8 == 0;
+}
+
 template <typename TT, int RB, int MB>
 int run_tk(itcpd_plan* p, const Side& sd, double* out, int64_t out_ld, int64_t nz,
            int64_t kk_per, int64_t kk_total, int64_t nsteps_i, int64_t split_stride) {
     constexpr int U = 2;
+    if (sizeof(TT) == 8 && RB == 4 && tk_ring_ok(p, sd) && kk_per % 2 == 0) {
+        dim3 grid((unsigned)(p->ML / 256), (unsigned)(p->Rp / 64), (unsigned)nz);
+        hipLaunchKernelGGL(k_tk_ring, grid, dim3(256), TKR_LDS, p->stream, (const double*)p->T, p->ML, sd,
+                           nsteps_i, kk_per, kk_total, (int)p->Rp, (int)p->R, out, out_ld, split_stride,
+                           p->chk);
+        ITCPD_HIP(hipGetLastError());
+        return ITCPD_OK;
+    }
     dim3 grid((unsigned)cdiv(p->ML, 64 * MB), (unsigned)(p->Rp / (16 * RB)), (unsigned)nz);
     const bool fast = (p->ML % (64 * MB) == 0) && (sd.dim[0] % 4 == 0) && (p->ML % 2 == 0);
     if (fast)
@@ -1079,7 +1216,8 @@ int launch_left_gemm(itcpd_plan* p, MSrc* ms) {
     const int64_t out_elems = p->ML * p->R;
     if (nz > 1 && (size_t)(nz * out_elems) > p->parts_elems) nz = (int64_t)(p->parts_elems / out_elems);
     if (nz < 1) nz = 1;
-    const int64_t kk_per = cdiv(kk_total, nz);
+    int64_t kk_per = cdiv(kk_total, nz);
+    if (tk_ring_ok(p, sd)) kk_per += kk_per & 1;   // whole stages of two 4-steps
     nz = cdiv(kk_total, kk_per);
     double* dst = (p->s > 1) ? p->PL : p->M;
     double* out = nz > 1 ? p->parts : dst;
