@@ -989,6 +989,119 @@ __global__ void k_sum_small(const double* __restrict__ part, int n, double* __re
 // ----------------------------------------------------------------- launchers
 namespace {
 
+// ------------------------------------------------- TR GEMM, LDS ring form
+// out(mR, r) = sum_{mL} T(mL, mR) KRP(mL, r) for a left side of one or two modes,
+// mL = i + I0 rest, KRP(mL, r) = A0(i, r) w(rest, r) with w = 1 (one mode) or
+// At1(rest, r) (two modes).  FP64 T, 64-column x 64-rank tiles, 512 threads.  A
+// stage is 32 consecutive i of one rest: T (64 columns x 32 i), A0 (64 ranks x
+// 32 i) -- a 256-byte run per column / rank -- and the 64 weights of the rest
+// land in an LDS slot through global_load_lds_dwordx4.  The 16-byte chunk q of
+// run j is stored at chunk position q ^ (j & 15), so the 16 lanes that read chunk
+// q of 16 consecutive runs hit distinct banks.  Four slots, three stages in
+// flight.  Workgroup z takes stages [z kk_per, (z+1) kk_per) of the (rest, i)
+// range.  Wave w computes rank block w & 3 for column half w >> 2 (two 16-column
+// tiles); over a stage lane (c, g) covers i = 8 g .. 8 g + 7 (k-step s takes
+// i = 8 g + s for all four g).
+constexpr int TRR_NS = 4;
+constexpr int TRR_RUN = 256;
+constexpr int TRR_HALF = 64 * TRR_RUN;
+constexpr int TRR_SLOT = 2 * TRR_HALF + 1024;
+constexpr int TRR_LDS = TRR_NS * TRR_SLOT;
+
+__global__ __launch_bounds__(512, 1) void k_tr_ring(const double* __restrict__ T, int64_t ML, int64_t MR,
+                                                    int64_t I0, const double* __restrict__ A0,
+                                                    const double* __restrict__ At1, int Rp,
+                                                    int64_t kk_per, int64_t kk_total, int R,
+                                                    double* __restrict__ out, int64_t split_stride,
+                                                    const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char ring[];
+    const int t = threadIdx.x, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int c = lane & 15, g = lane >> 4;
+    const int rb = wave & 3, ch = wave >> 2;
+    const int64_t n0 = (int64_t)blockIdx.x * 64;
+    const int r0 = blockIdx.y * 64;
+    const int64_t nstg = I0 / 32;
+    const int64_t q0 = (int64_t)blockIdx.z * kk_per;
+    const int64_t q1 = q0 + kk_per < kk_total ? q0 + kk_per : kk_total;
+    const int nst = (int)(q1 - q0);
+    const bool weighted = At1 != nullptr;
+    // loads per stage issued by this wave
+    const int nper = (weighted && wave == 0) ? 5 : 4;
+
+    // stage q into slot q % TRR_NS: wave w loads runs 8w .. 8w+7 of T (columns)
+    // and of A0 (ranks), four runs per instruction; wave 0 also the weights
+    auto issue = [&](int q) {
+        const int64_t qq = q0 + q;
+        const int64_t rest = qq / nstg;
+        const int64_t i0 = (qq - rest * nstg) * 32;
+        unsigned char* slot = ring + (q % TRR_NS) * TRR_SLOT;
+        const int jj = lane >> 4, pos = lane & 15;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int j0 = 8 * wave + 4 * k, j = j0 + jj;
+            const int qc = pos ^ (j & 15);
+            lds_dma16(T + (n0 + j) * ML + I0 * rest + i0 + 2 * qc, slot + j0 * TRR_RUN);
+            lds_dma16(A0 + (int64_t)(r0 + j) * I0 + i0 + 2 * qc, slot + TRR_HALF + j0 * TRR_RUN);
+        }
+        if (weighted && wave == 0)
+            lds_dma16(At1 + rest * Rp + r0 + 2 * (lane & 31), slot + 2 * TRR_HALF);
+    };
+
+    d4 acc[2];
+    acc[0] = acc[1] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < TRR_NS - 1 && q < nst; ++q) issue(q);
+    for (int q = 0; q < nst; ++q) {
+        // this wave's loads of stage q are done once at most those of the (up to
+        // two) later stages are pending
+        const int later = nst - 1 - q;
+        if (later <= 0) TKR_WAIT(0);
+        else if (later == 1) { if (nper == 5) TKR_WAIT(5); else TKR_WAIT(4); }
+        else { if (nper == 5) TKR_WAIT(10); else TKR_WAIT(8); }
+        __asm__ volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        __asm__ volatile("" ::: "memory");
+        if (q + TRR_NS - 1 < nst) issue(q + TRR_NS - 1);
+        const unsigned char* slot = ring + (q % TRR_NS) * TRR_SLOT;
+        const double wr = weighted ? reinterpret_cast<const double*>(slot + 2 * TRR_HALF)[16 * rb + c] : 1.0;
+        double kv[8], tv[2][8];
+        {
+            const unsigned char* kr = slot + TRR_HALF + (16 * rb + c) * TRR_RUN;
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const dv2 x = *reinterpret_cast<const dv2*>(kr + 16 * ((4 * g + h) ^ c));
+                kv[2 * h] = x.x;
+                kv[2 * h + 1] = x.y;
+            }
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const unsigned char* tr = slot + (32 * ch + 16 * b + c) * TRR_RUN;
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const dv2 x = *reinterpret_cast<const dv2*>(tr + 16 * ((4 * g + h) ^ c));
+                    tv[b][2 * h] = x.x;
+                    tv[b][2 * h + 1] = x.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int sk = 0; sk < 8; ++sk)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[b] = mfma_f64(kv[sk] * wr, tv[b][sk], acc[b]);
+    }
+    double* o = out + (int64_t)blockIdx.z * split_stride;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const int64_t col = n0 + 32 * ch + 16 * b + c;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + 16 * rb + g + 4 * v;
+            if (r < R) o[col + MR * r] = acc[b][v];
+        }
+    }
+}
+
 // contiguous run per lane in the TR kernel (reduction indices per 4-lane group = 4 * TR_RUN)
 constexpr int TR_RUN = 2;
 
@@ -1238,6 +1351,31 @@ int launch_left_gemm(itcpd_plan* p, MSrc* ms) {
 int launch_right_gemm(itcpd_plan* p, MSrc* ms) {
     if (p->prec != ITCPD_FP64) return launch_right_gemm_lp(p, ms);
     const Side sd = make_side(p, 0, p->s, /*rank_fast=*/false);
+    if (p->s <= 2 && p->elt_bytes == 8 && sd.dim[0] % 32 == 0 && p->MR % 64 == 0 && p->Rp % 64 == 0) {
+        // left side of one or two modes: the LDS ring form (one workgroup per CU),
+        // split over the (rest, i) stages to fill the chip
+        const int64_t ntx = p->MR / 64;
+        const int gy = (int)(p->Rp / 64);
+        const int64_t kk_total = (sd.dim[0] / 32) * (p->ML / sd.dim[0]);
+        const int64_t out_elems = p->MR * p->R;
+        int64_t nz = pick_split(ntx * gy, kk_total, 256, 8);
+        if (nz > 1 && (size_t)(nz * out_elems) > p->parts_elems) nz = (int64_t)(p->parts_elems / out_elems);
+        if (nz < 1) nz = 1;
+        const int64_t kk_per = cdiv(kk_total, nz);
+        nz = cdiv(kk_total, kk_per);
+        double* dst = (p->N - p->s > 1) ? p->PR : p->M;
+        double* out = nz > 1 ? p->parts : dst;
+        {
+            Timed tm(p, 1, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
+                     (double)p->ML * (double)p->MR * p->elt_bytes);
+            hipLaunchKernelGGL(k_tr_ring, dim3((unsigned)ntx, (unsigned)gy, (unsigned)nz), dim3(512), TRR_LDS,
+                               p->stream, (const double*)p->T, p->ML, p->MR, sd.dim[0],
+                               (const double*)sd.fac[0], p->s == 2 ? (const double*)p->At[1] : nullptr,
+                               (int)p->Rp, kk_per, kk_total, (int)p->R, out, out_elems, p->chk);
+            ITCPD_HIP(hipGetLastError());
+        }
+        return finish_parts(p, p->parts, nz, out_elems, dst, ms, p->N - p->s == 1);
+    }
     const int MB = mb_of(p->RB);
     constexpr int S = TR_RUN;
     const int64_t ngroups = cdiv(sd.dim[0], 4 * S);
