@@ -992,28 +992,45 @@ namespace {
 // ------------------------------------------------- TR GEMM, LDS ring form
 // out(mR, r) = sum_{mL} T(mL, mR) KRP(mL, r) for a left side of one or two modes,
 // mL = i + I0 rest, KRP(mL, r) = A0(i, r) w(rest, r) with w = 1 (one mode) or
-// At1(rest, r) (two modes).  FP64 T, 64-column x 64-rank tiles, 512 threads.  A
-// stage is 32 consecutive i of one rest: T (64 columns x 32 i), A0 (64 ranks x
-// 32 i) -- a 256-byte run per column / rank -- and the 64 weights of the rest
-// land in an LDS slot through global_load_lds_dwordx4.  The 16-byte chunk q of
-// run j is stored at chunk position q ^ (j & 15), so the 16 lanes that read chunk
-// q of 16 consecutive runs hit distinct banks.  Four slots, three stages in
-// flight.  Workgroup z takes stages [z kk_per, (z+1) kk_per) of the (rest, i)
-// range.  Wave w computes rank block w & 3 for column half w >> 2 (two 16-column
-// tiles); over a stage lane (c, g) covers i = 8 g .. 8 g + 7 (k-step s takes
-// i = 8 g + s for all four g).
-constexpr int TRR_NS = 4;
+// At1(rest, r) (two modes).  FP64 T, 64-column x 64-rank tiles.  A stage is 32
+// consecutive i of one rest: T (64 columns x 32 i), A0 (64 ranks x 32 i) -- a
+// 256-byte run per column / rank -- and the 64 weights of the rest land in an
+// LDS slot through global_load_lds_dwordx4.  The 16-byte chunk q of run j is
+// stored at chunk position q ^ (j & 15), so the 16 lanes that read chunk q of 16
+// consecutive runs hit distinct banks.  Workgroup z takes stages
+// [z kk_per, (z+1) kk_per) of the (rest, i) range.  Wave w computes rank block
+// w & 3 for its share of the 16-column tiles; over a stage lane (c, g) covers
+// i = 8 g .. 8 g + 7 (k-step s takes i = 8 g + s for all four g).  Measured at
+// 512^3, rank 64: four waves and two slots with two workgroups per CU (340 us)
+// beat eight waves and four slots with one (389 us) -- a workgroup waiting at
+// its barrier leaves the matrix cores to the other one.
 constexpr int TRR_RUN = 256;
 constexpr int TRR_HALF = 64 * TRR_RUN;
 constexpr int TRR_SLOT = 2 * TRR_HALF + 1024;
-constexpr int TRR_LDS = TRR_NS * TRR_SLOT;
 
-__global__ __launch_bounds__(512, 1) void k_tr_ring(const double* __restrict__ T, int64_t ML, int64_t MR,
-                                                    int64_t I0, const double* __restrict__ A0,
-                                                    const double* __restrict__ At1, int Rp,
-                                                    int64_t kk_per, int64_t kk_total, int R,
-                                                    double* __restrict__ out, int64_t split_stride,
-                                                    const DevCheck* __restrict__ chk) {
+// s_waitcnt vmcnt <= n (0 <= n < 16), lgkmcnt = 0
+__device__ __forceinline__ void wait_vm(int n) {
+    switch (n) {
+        case 0: TKR_WAIT(0); break;   case 1: TKR_WAIT(1); break;   case 2: TKR_WAIT(2); break;
+        case 3: TKR_WAIT(3); break;   case 4: TKR_WAIT(4); break;   case 5: TKR_WAIT(5); break;
+        case 6: TKR_WAIT(6); break;   case 7: TKR_WAIT(7); break;   case 8: TKR_WAIT(8); break;
+        case 9: TKR_WAIT(9); break;   case 10: TKR_WAIT(10); break; case 11: TKR_WAIT(11); break;
+        case 12: TKR_WAIT(12); break; case 13: TKR_WAIT(13); break; case 14: TKR_WAIT(14); break;
+        default: TKR_WAIT(15); break;
+    }
+}
+
+// NW waves (8: two 16-column tiles per wave, 4: four), NS slots
+template <int NW, int NS>
+__global__ __launch_bounds__(NW * 64, 8 / NW) void k_tr_ring(const double* __restrict__ T, int64_t ML,
+                                                             int64_t MR, int64_t I0,
+                                                             const double* __restrict__ A0,
+                                                             const double* __restrict__ At1, int Rp,
+                                                             int64_t kk_per, int64_t kk_total, int R,
+                                                             double* __restrict__ out, int64_t split_stride,
+                                                             const DevCheck* __restrict__ chk) {
+    constexpr int RPW = 64 / NW;      // runs of T (and of A0) loaded per wave
+    constexpr int NCT = 16 / NW;      // 16-column tiles per wave
     if (chk->done) return;
     extern __shared__ __attribute__((aligned(16))) unsigned char ring[];
     const int t = threadIdx.x, lane = t & 63;
@@ -1027,20 +1044,19 @@ __global__ __launch_bounds__(512, 1) void k_tr_ring(const double* __restrict__ T
     const int64_t q1 = q0 + kk_per < kk_total ? q0 + kk_per : kk_total;
     const int nst = (int)(q1 - q0);
     const bool weighted = At1 != nullptr;
-    // loads per stage issued by this wave
-    const int nper = (weighted && wave == 0) ? 5 : 4;
+    const int nper = 2 * (RPW / 4) + ((weighted && wave == 0) ? 1 : 0);
 
-    // stage q into slot q % TRR_NS: wave w loads runs 8w .. 8w+7 of T (columns)
-    // and of A0 (ranks), four runs per instruction; wave 0 also the weights
+    // stage q into slot q % NS: wave w loads runs RPW w .. RPW (w+1) - 1 of T
+    // (columns) and of A0 (ranks), four runs per instruction; wave 0 also the weights
     auto issue = [&](int q) {
         const int64_t qq = q0 + q;
         const int64_t rest = qq / nstg;
         const int64_t i0 = (qq - rest * nstg) * 32;
-        unsigned char* slot = ring + (q % TRR_NS) * TRR_SLOT;
+        unsigned char* slot = ring + (q % NS) * TRR_SLOT;
         const int jj = lane >> 4, pos = lane & 15;
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int j0 = 8 * wave + 4 * k, j = j0 + jj;
+        for (int k = 0; k < RPW / 4; ++k) {
+            const int j0 = RPW * wave + 4 * k, j = j0 + jj;
             const int qc = pos ^ (j & 15);
             lds_dma16(T + (n0 + j) * ML + I0 * rest + i0 + 2 * qc, slot + j0 * TRR_RUN);
             lds_dma16(A0 + (int64_t)(r0 + j) * I0 + i0 + 2 * qc, slot + TRR_HALF + j0 * TRR_RUN);
@@ -1049,34 +1065,33 @@ __global__ __launch_bounds__(512, 1) void k_tr_ring(const double* __restrict__ T
             lds_dma16(At1 + rest * Rp + r0 + 2 * (lane & 31), slot + 2 * TRR_HALF);
     };
 
-    d4 acc[2];
-    acc[0] = acc[1] = (d4){0.0, 0.0, 0.0, 0.0};
-    for (int q = 0; q < TRR_NS - 1 && q < nst; ++q) issue(q);
+    d4 acc[NCT];
+#pragma unroll
+    for (int b = 0; b < NCT; ++b) acc[b] = (d4){0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < NS - 1 && q < nst; ++q) issue(q);
     for (int q = 0; q < nst; ++q) {
-        // this wave's loads of stage q are done once at most those of the (up to
-        // two) later stages are pending
-        const int later = nst - 1 - q;
-        if (later <= 0) TKR_WAIT(0);
-        else if (later == 1) { if (nper == 5) TKR_WAIT(5); else TKR_WAIT(4); }
-        else { if (nper == 5) TKR_WAIT(10); else TKR_WAIT(8); }
+        // this wave's loads of stage q are done once at most those of the later
+        // stages already requested are pending
+        const int later = (nst - 1 - q) < (NS - 2) ? (nst - 1 - q) : (NS - 2);
+        wait_vm(later * nper);
         __asm__ volatile("" ::: "memory");
         __builtin_amdgcn_s_barrier();
         __asm__ volatile("" ::: "memory");
-        if (q + TRR_NS - 1 < nst) issue(q + TRR_NS - 1);
-        const unsigned char* slot = ring + (q % TRR_NS) * TRR_SLOT;
+        if (q + NS - 1 < nst) issue(q + NS - 1);
+        const unsigned char* slot = ring + (q % NS) * TRR_SLOT;
         const double wr = weighted ? reinterpret_cast<const double*>(slot + 2 * TRR_HALF)[16 * rb + c] : 1.0;
-        double kv[8], tv[2][8];
+        double kv[8], tv[NCT][8];
         {
             const unsigned char* kr = slot + TRR_HALF + (16 * rb + c) * TRR_RUN;
 #pragma unroll
             for (int h = 0; h < 4; ++h) {
                 const dv2 x = *reinterpret_cast<const dv2*>(kr + 16 * ((4 * g + h) ^ c));
-                kv[2 * h] = x.x;
-                kv[2 * h + 1] = x.y;
+                kv[2 * h] = x.x * wr;
+                kv[2 * h + 1] = x.y * wr;
             }
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                const unsigned char* tr = slot + (32 * ch + 16 * b + c) * TRR_RUN;
+            for (int b = 0; b < NCT; ++b) {
+                const unsigned char* tr = slot + (16 * NCT * ch + 16 * b + c) * TRR_RUN;
 #pragma unroll
                 for (int h = 0; h < 4; ++h) {
                     const dv2 x = *reinterpret_cast<const dv2*>(tr + 16 * ((4 * g + h) ^ c));
@@ -1088,12 +1103,12 @@ __global__ __launch_bounds__(512, 1) void k_tr_ring(const double* __restrict__ T
 #pragma unroll
         for (int sk = 0; sk < 8; ++sk)
 #pragma unroll
-            for (int b = 0; b < 2; ++b) acc[b] = mfma_f64(kv[sk] * wr, tv[b][sk], acc[b]);
+            for (int b = 0; b < NCT; ++b) acc[b] = mfma_f64(kv[sk], tv[b][sk], acc[b]);
     }
     double* o = out + (int64_t)blockIdx.z * split_stride;
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        const int64_t col = n0 + 32 * ch + 16 * b + c;
+    for (int b = 0; b < NCT; ++b) {
+        const int64_t col = n0 + 16 * NCT * ch + 16 * b + c;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int r = r0 + 16 * rb + g + 4 * v;
@@ -1101,6 +1116,9 @@ __global__ __launch_bounds__(512, 1) void k_tr_ring(const double* __restrict__ T
         }
     }
 }
+// the configuration in use: waves per workgroup, LDS slots, workgroups per CU
+constexpr int TRR_NW = 4, TRR_NS = 2, TRR_WG_PER_CU = 2;
+constexpr int TRR_LDS = TRR_NS * TRR_SLOT;
 
 // contiguous run per lane in the TR kernel (reduction indices per 4-lane group = 4 * TR_RUN)
 constexpr int TR_RUN = 2;
@@ -1358,7 +1376,7 @@ int launch_right_gemm(itcpd_plan* p, MSrc* ms) {
         const int gy = (int)(p->Rp / 64);
         const int64_t kk_total = (sd.dim[0] / 32) * (p->ML / sd.dim[0]);
         const int64_t out_elems = p->MR * p->R;
-        int64_t nz = pick_split(ntx * gy, kk_total, 256, 8);
+        int64_t nz = pick_split(ntx * gy, kk_total, 256 * TRR_WG_PER_CU, 8);
         if (nz > 1 && (size_t)(nz * out_elems) > p->parts_elems) nz = (int64_t)(p->parts_elems / out_elems);
         if (nz < 1) nz = 1;
         const int64_t kk_per = cdiv(kk_total, nz);
@@ -1368,7 +1386,8 @@ int launch_right_gemm(itcpd_plan* p, MSrc* ms) {
         {
             Timed tm(p, 1, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
                      (double)p->ML * (double)p->MR * p->elt_bytes);
-            hipLaunchKernelGGL(k_tr_ring, dim3((unsigned)ntx, (unsigned)gy, (unsigned)nz), dim3(512), TRR_LDS,
+            hipLaunchKernelGGL((k_tr_ring<TRR_NW, TRR_NS>), dim3((unsigned)ntx, (unsigned)gy, (unsigned)nz),
+                               dim3(TRR_NW * 64), TRR_LDS,
                                p->stream, (const double*)p->T, p->ML, p->MR, sd.dim[0],
                                (const double*)sd.fac[0], p->s == 2 ? (const double*)p->At[1] : nullptr,
                                (int)p->Rp, kk_per, kk_total, (int)p->R, out, out_elems, p->chk);
