@@ -253,8 +253,11 @@ __global__ __launch_bounds__(256, 2) void k_tk_ring(const double* __restrict__ T
         double ta[U][MB], ka[U][RB];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const dv2* tp = reinterpret_cast<const dv2*>(slot + (4 * u + g) * TKR_CST + (wave * 64 + 4 * c) * 8);
-            const dv2 x0 = tp[0], x1 = tp[1];
+            // rows 2c, 2c+1 and 32+2c, 33+2c of the wave's 64: eight lanes read 128
+            // contiguous bytes per instruction
+            const unsigned char* tp = slot + (4 * u + g) * TKR_CST + (wave * 64 + 2 * c) * 8;
+            const dv2 x0 = *reinterpret_cast<const dv2*>(tp);
+            const dv2 x1 = *reinterpret_cast<const dv2*>(tp + 256);
             ta[u][0] = x0.x; ta[u][1] = x0.y; ta[u][2] = x1.x; ta[u][3] = x1.y;
             const double* kp = reinterpret_cast<const double*>(slot + TKR_TBYTES) + (4 * u + g) * 64 + c;
 #pragma unroll
@@ -269,7 +272,7 @@ __global__ __launch_bounds__(256, 2) void k_tk_ring(const double* __restrict__ T
     }
 
     double* o = out + (int64_t)blockIdx.z * split_stride;
-    const int64_t mrow = mwg + wave * 64 + MB * c;
+    const int64_t mrow = mwg + wave * 64 + 2 * c;
 #pragma unroll
     for (int a = 0; a < RB; ++a)
 #pragma unroll
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(256, 2) void k_tk_ring(const double* __restrict__ T
             const int r = r0 + 16 * a + g + 4 * v;
             if (r >= R) continue;
 #pragma unroll
-            for (int b = 0; b < MB; ++b) o[mrow + b + out_ld * r] = acc[a][b][v];
+            for (int b = 0; b < MB; ++b) o[mrow + 32 * (b >> 1) + (b & 1) + out_ld * r] = acc[a][b][v];
         }
 }
 
