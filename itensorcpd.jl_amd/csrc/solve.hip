@@ -319,15 +319,12 @@ __global__ __launch_bounds__(256) void k_factor64(GramSet gs, int mode, int R, d
 // X is written to global memory and to an LDS tile whose X^T X is accumulated
 // into gram_part[b].  The QR fallback applies Q^T, T11^{-1}, Z^T row by row.
 template <int RQ, int RPW>
-__global__ __launch_bounds__(256) void k_solve(
+__device__ __forceinline__ void solve_rows_dev(
     const double* __restrict__ Msrc, int64_t In, int R,
     const double* __restrict__ Ug, const double* __restrict__ invdiag, const int* __restrict__ piv,
     const int* __restrict__ finfo, const double* __restrict__ W, const double* __restrict__ qtau,
     const double* __restrict__ ztau, const int* __restrict__ jpvt, int u_in_lds,
-    double* __restrict__ X, double* __restrict__ Msum, double* __restrict__ gram_part, int grams,
-    int only_fallback, const DevCheck* __restrict__ chk) {
-    if (chk->done) return;
-    if (only_fallback && finfo[1] == 0) return;   // the Cholesky case went to k_solve_mfma
+    double* __restrict__ X, double* __restrict__ Msum, double* __restrict__ gram_part, int grams) {
     constexpr int RPB = 4 * RPW;
     extern __shared__ double lds[];
     const int t = threadIdx.x;
@@ -459,30 +456,55 @@ __global__ __launch_bounds__(256) void k_solve(
     }
 }
 
-// Row solves for R <= RT <= 64 with the explicit V^-1 the factor kernel formed:
-// X = M V^-1 on FP64 MFMA.  A block of RT/16 waves takes 16-row tiles (grid
-// stride); wave w owns output columns 16w..16w+15 and keeps its V^-1 column panel
-// in registers for all tiles.  f64 16x16x4 layout: A lane (c, g) = A[c][k0+g],
-// B lane = B[k0+g][c], result v = D[g+4v][c].  The wave's result tile is, as it
-// stands, the A operand of X^T X over the tile's rows (k0 = 4v), so the block's
-// Gram partial needs only the other waves' tiles from LDS; it stays in registers
+template <int RQ, int RPW>
+__global__ __launch_bounds__(256) void k_solve(
+    const double* __restrict__ Msrc, int64_t In, int R,
+    const double* __restrict__ Ug, const double* __restrict__ invdiag, const int* __restrict__ piv,
+    const int* __restrict__ finfo, const double* __restrict__ W, const double* __restrict__ qtau,
+    const double* __restrict__ ztau, const int* __restrict__ jpvt, int u_in_lds,
+    double* __restrict__ X, double* __restrict__ Msum, double* __restrict__ gram_part, int grams,
+    const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    solve_rows_dev<RQ, RPW>(Msrc, In, R, Ug, invdiag, piv, finfo, W, qtau, ztau, jpvt, u_in_lds, X, Msum,
+                            gram_part, grams);
+}
+
+// Row solves for R <= RT <= 64.  Cholesky case: X = M V^-1 on FP64 MFMA with the
+// explicit V^-1 the factor kernel formed.  Blocks of 256 threads take 16-row
+// tiles (grid stride); wave w < RT/16 owns output columns 16w..16w+15 and keeps
+// its V^-1 column panel in registers for all tiles (the other waves only join the
+// barriers).  f64 16x16x4 layout: A lane (c, g) = A[c][k0+g], B lane =
+// B[k0+g][c], result v = D[g+4v][c].  The wave's result tile is, as it stands,
+// the A operand of X^T X over the tile's rows (k0 = 4v), so the block's Gram
+// partial needs only the other waves' tiles from LDS; it stays in registers
 // across tiles and is written once.  All loads of a tile are issued before the
-// first MFMA.  M is copied to Msum when asked.  Exits at once when the factor
-// kernel took the QR fallback (k_solve handles that case).
+// first MFMA.  M is copied to Msum when asked.  QR fallback case: the same
+// blocks run the row-by-row fallback solve (16-row tiles as well, so the Gram
+// partials line up), with dynamic LDS for its 16 x R tile.
 template <int RT>
-__global__ __launch_bounds__(RT * 4) void k_solve_mfma(const double* __restrict__ Msrc, int64_t In, int R,
-                                                       const double* __restrict__ Vinv,
-                                                       const int* __restrict__ finfo,
-                                                       double* __restrict__ X, double* __restrict__ Msum,
-                                                       double* __restrict__ gram_part, int grams,
-                                                       const DevCheck* __restrict__ chk) {
-    if (chk->done || finfo[1] != 0) return;
+__global__ __launch_bounds__(256) void k_solve_mfma(const double* __restrict__ Msrc, int64_t In, int R,
+                                                    const double* __restrict__ Vinv,
+                                                    const int* __restrict__ finfo,
+                                                    const double* __restrict__ W,
+                                                    const double* __restrict__ qtau,
+                                                    const double* __restrict__ ztau,
+                                                    const int* __restrict__ jpvt,
+                                                    double* __restrict__ X, double* __restrict__ Msum,
+                                                    double* __restrict__ gram_part, int grams,
+                                                    const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    if (finfo[1] != 0) {
+        solve_rows_dev<1, 4>(Msrc, In, R, nullptr, nullptr, nullptr, finfo, W, qtau, ztau, jpvt, 0, X, Msum,
+                             gram_part, grams);
+        return;
+    }
     constexpr int NT = RT / 16, KS = RT / 4, LDX = RT + 2;
     __shared__ double Xs[16 * LDX];
     const int t = threadIdx.x, l = t & 63, c = l & 15, g = l >> 4;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const bool active = w < NT;
     const int col = 16 * w + c;
-    const bool colok = col < R;
+    const bool colok = active && col < R;
     double bv[KS];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
@@ -493,43 +515,49 @@ __global__ __launch_bounds__(RT * 4) void k_solve_mfma(const double* __restrict_
 #pragma unroll
     for (int tb = 0; tb < NT; ++tb) gacc[tb] = (d4){0.0, 0.0, 0.0, 0.0};
     for (int64_t i0 = (int64_t)blockIdx.x * 16; i0 < In; i0 += (int64_t)gridDim.x * 16) {
-        const int64_t ia = i0 + c;
-        const bool aok = ia < In;
-        double av[KS];
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const int k = 4 * ks + g;
-            av[ks] = (aok && k < R) ? Msrc[ia + In * k] : 0.0;
-        }
-        if (Msum) {
-#pragma unroll
-            for (int ks = 4 * w; ks < 4 * w + 4; ++ks) {
-                const int k = 4 * ks + g;
-                if (aok && k < R) Msum[ia + In * k] = av[ks];
-            }
-        }
         d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+        if (active) {
+            const int64_t ia = i0 + c;
+            const bool aok = ia < In;
+            double av[KS];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) acc = mfma_f64(av[ks], bv[ks], acc);
+            for (int ks = 0; ks < KS; ++ks) {
+                const int k = 4 * ks + g;
+                av[ks] = (aok && k < R) ? Msrc[ia + In * k] : 0.0;
+            }
+            if (Msum) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int64_t row = i0 + g + 4 * v;
-            if (row < In && colok) X[row + In * col] = acc[v];
+                for (int ks = 4 * w; ks < 4 * w + 4; ++ks) {
+                    const int k = 4 * ks + g;
+                    if (aok && k < R) Msum[ia + In * k] = av[ks];
+                }
+            }
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) acc = mfma_f64(av[ks], bv[ks], acc);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = i0 + g + 4 * v;
+                if (row < In && colok) X[row + In * col] = acc[v];
+            }
         }
         if (grams) {
             // rows >= In and columns >= R of acc are zero (their operands were)
+            if (active) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Xs[(g + 4 * v) * LDX + col] = acc[v];
+                for (int v = 0; v < 4; ++v) Xs[(g + 4 * v) * LDX + col] = acc[v];
+            }
             __syncthreads();
+            if (active) {
 #pragma unroll
-            for (int tb = 0; tb < NT; ++tb)
+                for (int tb = 0; tb < NT; ++tb)
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    gacc[tb] = mfma_f64(acc[v], Xs[(4 * v + g) * LDX + 16 * tb + c], gacc[tb]);
+                    for (int v = 0; v < 4; ++v)
+                        gacc[tb] = mfma_f64(acc[v], Xs[(4 * v + g) * LDX + 16 * tb + c], gacc[tb]);
+            }
             __syncthreads();
         }
     }
-    if (grams) {
+    if (grams && active) {
         double* gp = gram_part + (int64_t)blockIdx.x * R * R;
 #pragma unroll
         for (int tb = 0; tb < NT; ++tb)
@@ -974,27 +1002,23 @@ int run_solve_k(itcpd_plan* p, int mode, const double* Msrc, int nparts, int64_t
     double* W = p->qr_aux;
     hipLaunchKernelGGL((k_solve<RQ, RPW>), dim3(nb), dim3(256), lds, p->stream, Msrc, In, R, p->U, p->invdiag, p->piv, p->finfo, W, W + (size_t)R * R,
                        W + (size_t)R * R + R, (const int*)(p->piv + R), u_lds ? 1 : 0, p->X,
-                       keep_m ? p->Msum : (double*)nullptr, p->gram_part, grams ? 1 : 0, 0, p->chk);
+                       keep_m ? p->Msum : (double*)nullptr, p->gram_part, grams ? 1 : 0, p->chk);
     ITCPD_HIP(hipGetLastError());
     return ITCPD_OK;
 }
 
-// R <= 64: MFMA product with V^-1 for the Cholesky case, k_solve for the QR fallback
-// (each exits at once in the other case); both use the same nb row blocks.
+// R <= 64: one launch; the kernel takes the Cholesky (MFMA) or the QR-fallback path
+// from the factor kernel's status.
 template <int RT>
 int run_solve_rows(itcpd_plan* p, int mode, const double* Msrc, bool grams, bool keep_m) {
     const int64_t In = p->dims[mode];
     const int R = (int)p->R;
     const int nb = solve_nb(In, R);
     double* Msum = keep_m ? p->Msum : (double*)nullptr;
-    hipLaunchKernelGGL((k_solve_mfma<RT>), dim3(nb), dim3(RT * 4), 0, p->stream, Msrc, In, R, p->Upad, p->finfo,
-                       p->X, Msum, p->gram_part, grams ? 1 : 0, p->chk);
-    ITCPD_HIP(hipGetLastError());
     double* W = p->qr_aux;
-    const size_t lds = (size_t)16 * R * 8 + (size_t)R * (R + 1) * 8;
-    hipLaunchKernelGGL((k_solve<1, 4>), dim3(nb), dim3(256), lds, p->stream, Msrc, In, R, p->U, p->invdiag,
-                       p->piv, p->finfo, W, W + (size_t)R * R, W + (size_t)R * R + R,
-                       (const int*)(p->piv + R), 1, p->X, Msum, p->gram_part, grams ? 1 : 0, 1, p->chk);
+    hipLaunchKernelGGL((k_solve_mfma<RT>), dim3(nb), dim3(256), (size_t)16 * R * 8, p->stream, Msrc, In, R,
+                       p->Upad, p->finfo, W, W + (size_t)R * R, W + (size_t)R * R + R,
+                       (const int*)(p->piv + R), p->X, Msum, p->gram_part, grams ? 1 : 0, p->chk);
     ITCPD_HIP(hipGetLastError());
     return ITCPD_OK;
 }
