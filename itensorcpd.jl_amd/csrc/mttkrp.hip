@@ -207,12 +207,18 @@ __global__ __launch_bounds__(256, 2) void k_tk_ring(const double* __restrict__ T
     const int64_t I0 = sd.dim[0];
     const double* __restrict__ At0 = sd.fac[0];
 
-    // requests stage q into slot q % TKR_NS: wave w loads T columns 2w, 2w+1 (two
-    // 1 KB halves each) and At0 rows 2w, 2w+1 of the stage
+    // stage positions advance by U four-steps without divisions: (rest, step
+    // within the rest) of the next stage to request and of the stage to compute
+    int64_t rest_is = kk0 / nsteps_i, st_is = kk0 - rest_is * nsteps_i;
+    int64_t rest_c = rest_is;
+    int64_t st_c = st_is;
+    // requests the next stage into slot q % TKR_NS: wave w loads T columns 2w, 2w+1
+    // (two 1 KB halves each) and At0 rows 2w, 2w+1 of the stage
     auto issue = [&](int q) {
-        const int64_t kk = kk0 + (int64_t)q * U;
-        const int64_t rest = kk / nsteps_i;
-        const int64_t i0 = (kk - rest * nsteps_i) * 4;
+        const int64_t rest = rest_is;
+        const int64_t i0 = st_is * 4;
+        st_is += U;
+        if (st_is >= nsteps_i) { st_is -= nsteps_i; ++rest_is; }
         unsigned char* slot = ring + (q % TKR_NS) * TKR_SLOT;
 #pragma unroll
         for (int e2 = 0; e2 < 2; ++e2) {
@@ -243,7 +249,9 @@ __global__ __launch_bounds__(256, 2) void k_tk_ring(const double* __restrict__ T
         __builtin_amdgcn_s_barrier();
         __asm__ volatile("" ::: "memory");
         if (q + TKR_NS - 1 < nst) issue(q + TKR_NS - 1);
-        const int64_t rest = (kk0 + (int64_t)q * U) / nsteps_i;
+        const int64_t rest = rest_c;
+        st_c += U;
+        if (st_c >= nsteps_i) { st_c -= nsteps_i; ++rest_c; }
         if (rest != cur_rest) {
             cur_rest = rest;
 #pragma unroll
@@ -1049,12 +1057,14 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void k_tr_ring(const double* __res
     const bool weighted = At1 != nullptr;
     const int nper = 2 * (RPW / 4) + ((weighted && wave == 0) ? 1 : 0);
 
+    // (rest, 32-index chunk) of the next stage to request, advanced without divisions
+    int64_t rest_is = q0 / nstg, ic_is = q0 - rest_is * nstg;
     // stage q into slot q % NS: wave w loads runs RPW w .. RPW (w+1) - 1 of T
     // (columns) and of A0 (ranks), four runs per instruction; wave 0 also the weights
     auto issue = [&](int q) {
-        const int64_t qq = q0 + q;
-        const int64_t rest = qq / nstg;
-        const int64_t i0 = (qq - rest * nstg) * 32;
+        const int64_t rest = rest_is;
+        const int64_t i0 = ic_is * 32;
+        if (++ic_is == nstg) { ic_is = 0; ++rest_is; }
         unsigned char* slot = ring + (q % NS) * TRR_SLOT;
         const int jj = lane >> 4, pos = lane & 15;
 #pragma unroll
