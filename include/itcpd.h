@@ -47,23 +47,35 @@ typedef enum { ITCPD_ALG_KRP_FREE_NORMAL = 0, ITCPD_ALG_KRP_NORMAL = 1 } itcpd_a
 
 typedef enum { ITCPD_MEM_HOST = 0, ITCPD_MEM_DEVICE = 1 } itcpd_mem;
 
-typedef enum { ITCPD_CHECK_NONE = 0, ITCPD_CHECK_FIT = 1 } itcpd_check_kind;
+/* NoCheck, FitCheck, CPDiffCheck (converge_checks/cp_diff_check.jl:6-71) and
+ * CPAngleCheck (cp_angle_check.jl:6-73). */
+typedef enum {
+    ITCPD_CHECK_NONE = 0,
+    ITCPD_CHECK_FIT = 1,
+    ITCPD_CHECK_CPDIFF = 2,
+    ITCPD_CHECK_CPANGLE = 3
+} itcpd_check_kind;
 
-/* Convergence-check state, in/out.  Mirrors the mutable fields of
- * FitCheck (converge_checks/fit_check.jl:5-20) and NoCheck (no_check.jl:1-6). */
+/* Convergence-check state, in/out.  Mirrors the mutable fields of FitCheck
+ * (converge_checks/fit_check.jl:5-20), NoCheck (no_check.jl:1-6), CPDiffCheck
+ * (cp_diff_check.jl:6-18) and CPAngleCheck (cp_angle_check.jl:6-18).  For the two
+ * CP-vs-CP checks the previous sweep's CPD (check.PrevCP) stays on the device in
+ * the plan: has_prev = 1 on entry requires that the last itcpd_run on this plan
+ * left it there (ITCPD_E_STATE otherwise). */
 typedef struct {
     int32_t kind;          /* itcpd_check_kind */
     int32_t iter;          /* check.iter (in/out) */
-    int32_t counter;       /* FitCheck.counter (in/out) */
+    int32_t counter;       /* check.counter (in/out) */
     int32_t max_counter;   /* check.max_counter */
-    int32_t total_iter;    /* FitCheck.total_iter (out) */
+    int32_t total_iter;    /* check.total_iter (out) */
     int32_t sweeps_run;    /* out: sweeps executed by this itcpd_run call */
     int32_t converged;     /* out: 1 if check_converge returned true */
-    int32_t reserved;
-    double tolerance;      /* FitCheck.tolerance */
-    double ref_norm;       /* FitCheck.ref_norm = norm(T) */
-    double lastfit;        /* check.lastfit (in/out) */
-    double final_fit;      /* FitCheck.final_fit (out); CPDFit(check) */
+    int32_t has_prev;      /* CPDiff / CPAngle: PrevCP !== nothing (in/out) */
+    double tolerance;      /* check.tolerance */
+    double ref_norm;       /* FitCheck: ref_norm = norm(T); CPDiffCheck: norm_prev_iter
+                              (||X_prev||^2); CPAngleCheck: norm_prev_iter (||X_prev||) (in/out) */
+    double lastfit;        /* check.lastfit; CPAngleCheck: check.lastangle (in/out) */
+    double final_fit;      /* check.final_fit (out); CPDFit(check) */
 } itcpd_check;
 
 typedef struct itcpd_plan itcpd_plan;

@@ -7,21 +7,22 @@ a test written like the reference's ``test/cp_als.jl``) reads the same:
     decompose(A, epsilon, max_rank; ...)  -> decompose_adaptive decompose.jl:32-69
     random_CPD(A, rank; rng)                                   cpd.jl:63-82
     als_optimize(A, cp; alg, check, maxiter, verbose)          als_optimizer.jl:15-25
-    FitCheck(tol, max, norm), NoCheck(max), CPDFit(check)      converge_checks/*.jl
+    FitCheck(tol, max, norm), NoCheck(max), CPDiffCheck(tol, max),
+    CPAngleCheck(tol, max), CPDFit(check)                      converge_checks/*.jl
     KRPFreeNormal(), KRPNormal()                               algorithms/.../standard/tensor.jl
     CPD (factors, lam), cp_rank                                cpd.jl:7-46
 
 All arithmetic runs in libitcpd_hip.so on the GPU; there is no CPU fallback.
 """
 from .api import (  # noqa: F401
-    CPD, ALSPlan, BatchALS, DeviceTensor, MLP, reconstruct, CPDFit, FitCheck, KRPFreeNormal, KRPNormal, NoCheck, CPDOptimizer,
+    CPD, ALSPlan, BatchALS, DeviceTensor, MLP, reconstruct, CPDFit, FitCheck, CPDiffCheck, CPAngleCheck, KRPFreeNormal, KRPNormal, NoCheck, CPDOptimizer,
     als_optimize, cp_rank, decompose, decompose_adaptive, decompose_batch, increase_cpd_rank,
     random_CPD, tensor_norm,
 )
 from ._lib import ITCPDError  # noqa: F401
 
 __all__ = [
-    "CPD", "ALSPlan", "BatchALS", "decompose_batch", "DeviceTensor", "MLP", "reconstruct", "CPDFit", "FitCheck", "KRPFreeNormal", "KRPNormal", "NoCheck",
+    "CPD", "ALSPlan", "BatchALS", "decompose_batch", "DeviceTensor", "MLP", "reconstruct", "CPDFit", "FitCheck", "CPDiffCheck", "CPAngleCheck", "KRPFreeNormal", "KRPNormal", "NoCheck",
     "CPDOptimizer", "als_optimize", "cp_rank", "decompose", "decompose_adaptive",
     "increase_cpd_rank", "random_CPD", "tensor_norm", "ITCPDError",
 ]

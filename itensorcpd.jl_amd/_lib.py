@@ -17,7 +17,7 @@ ITCPD_E_UNSUPPORTED = -6
 ITCPD_FP64, ITCPD_FP32, ITCPD_BF16X3 = 0, 1, 2
 ITCPD_ALG_KRP_FREE_NORMAL, ITCPD_ALG_KRP_NORMAL = 0, 1
 ITCPD_MEM_HOST, ITCPD_MEM_DEVICE = 0, 1
-ITCPD_CHECK_NONE, ITCPD_CHECK_FIT = 0, 1
+ITCPD_CHECK_NONE, ITCPD_CHECK_FIT, ITCPD_CHECK_CPDIFF, ITCPD_CHECK_CPANGLE = 0, 1, 2, 3
 ITCPD_KERNEL_GEMM_LEFT, ITCPD_KERNEL_GEMM_RIGHT, ITCPD_KERNEL_GEMM_ALL = 0, 1, 2
 
 # Every function the header declares, with (restype, argtypes).
@@ -31,7 +31,7 @@ class CheckStruct(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("iter", C.c_int32), ("counter", C.c_int32),
         ("max_counter", C.c_int32), ("total_iter", C.c_int32), ("sweeps_run", C.c_int32),
-        ("converged", C.c_int32), ("reserved", C.c_int32),
+        ("converged", C.c_int32), ("has_prev", C.c_int32),
         ("tolerance", C.c_double), ("ref_norm", C.c_double),
         ("lastfit", C.c_double), ("final_fit", C.c_double),
     ]

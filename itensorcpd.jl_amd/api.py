@@ -65,8 +65,41 @@ class FitCheck:
         self.total_iter = 0
 
 
-def CPDFit(check: FitCheck) -> float:
-    """fit_check.jl:68."""
+class CPDiffCheck:
+    """cp_diff_check.jl:6-18: stop after two consecutive small changes of
+    1 - ||X_n - X_{n-1}|| / ||X_{n-1}|| between consecutive sweeps.  PrevCP lives on
+    the device in the plan that ran the last sweep; ``has_prev`` marks it."""
+
+    def __init__(self, tol: float, max_counter: int):
+        self.iter = 0
+        self.counter = 0
+        self.tolerance = float(tol)
+        self.max_counter = int(max_counter)
+        self.norm_prev_iter = 0.0
+        self.has_prev = False
+        self.lastfit = 1.0
+        self.final_fit = 0.0
+        self.total_iter = 0
+
+
+class CPAngleCheck:
+    """cp_angle_check.jl:6-18: stop after two consecutive small changes of the angle
+    acos(<X_n, X_{n-1}> / (||X_n|| ||X_{n-1}||)) between consecutive sweeps."""
+
+    def __init__(self, tol: float, max_counter: int):
+        self.iter = 0
+        self.counter = 0
+        self.tolerance = float(tol)
+        self.max_counter = int(max_counter)
+        self.norm_prev_iter = 0.0
+        self.has_prev = False
+        self.lastangle = 1.0
+        self.final_fit = 0.0
+        self.total_iter = 0
+
+
+def CPDFit(check) -> float:
+    """fit_check.jl:68, cp_diff_check.jl:73, cp_angle_check.jl:75."""
     return check.final_fit
 
 
@@ -335,12 +368,21 @@ class ALSPlan:
             cs.final_fit = check.final_fit
         elif isinstance(check, NoCheck):
             cs.kind = L.ITCPD_CHECK_NONE
+        elif isinstance(check, (CPDiffCheck, CPAngleCheck)):
+            cs.kind = L.ITCPD_CHECK_CPDIFF if isinstance(check, CPDiffCheck) else L.ITCPD_CHECK_CPANGLE
+            cs.counter = check.counter
+            cs.tolerance = check.tolerance
+            cs.ref_norm = check.norm_prev_iter
+            cs.has_prev = 1 if check.has_prev else 0
+            cs.total_iter = check.total_iter
+            cs.final_fit = check.final_fit
         else:
             raise NotImplementedError(f"check {type(check).__name__} is not on the GPU path")
+        angle = isinstance(check, CPAngleCheck)
         cs.iter = check.iter
         cs.max_counter = check.max_counter
-        cs.lastfit = check.lastfit
-        iter0, lastfit0 = check.iter, check.lastfit
+        cs.lastfit = check.lastangle if angle else check.lastfit
+        iter0, lastfit0 = check.iter, cs.lastfit
         nmax = max(check.max_counter - check.iter, 0)
         trace = np.zeros(max(nmax, 1))
         rc = self._lib.itcpd_run(self._h, C.byref(cs),
@@ -348,12 +390,18 @@ class ALSPlan:
         if rc not in (L.ITCPD_OK, L.ITCPD_E_NAN):
             L.check(rc)
         check.iter = cs.iter
-        check.lastfit = cs.lastfit
-        if isinstance(check, FitCheck):
+        if angle:
+            check.lastangle = cs.lastfit
+        else:
+            check.lastfit = cs.lastfit
+        if not isinstance(check, NoCheck):
             check.counter = cs.counter
             check.total_iter = cs.total_iter
             check.final_fit = cs.final_fit
-        fits = trace[:cs.sweeps_run].copy() if isinstance(check, FitCheck) else np.zeros(0)
+        if isinstance(check, (CPDiffCheck, CPAngleCheck)):
+            check.norm_prev_iter = cs.ref_norm
+            check.has_prev = bool(cs.has_prev)
+        fits = trace[:cs.sweeps_run].copy() if not isinstance(check, NoCheck) else np.zeros(0)
         if verbose:
             _print_trace(check, self.rank, iter0, lastfit0, fits, cs.sweeps_run)
         if rc == L.ITCPD_E_NAN:
@@ -419,10 +467,14 @@ class ALSPlan:
 
 
 def _print_trace(check, R, iter0, lastfit0, fits, nsweeps):
-    """The verbose lines of fit_check.jl:36-38 / no_check.jl:12-14."""
-    if isinstance(check, FitCheck):
+    """The verbose lines of fit_check.jl:36-38 / no_check.jl:12-14 /
+    cp_diff_check.jl:43-45 / cp_angle_check.jl:45-47 (the CP-vs-CP checks print
+    nothing on the sweep that only stores PrevCP, marked NaN in the trace)."""
+    if not isinstance(check, NoCheck):
         last = lastfit0
         for k, f in enumerate(fits):
+            if f != f:
+                continue
             print(f"{R}\t {iter0 + k + 1} \t {f} \t {abs(last - f)}")
             last = f
     else:

@@ -314,6 +314,9 @@ int itcpd_plan_destroy(itcpd_plan* p) {
     dfree(p->scal);
     dfree(p->chk);
     dfree(p->fit_trace);
+    for (int q = 0; q < ITCPD_MAX_ORDER; ++q) dfree(p->Atprev[q]);
+    dfree(p->lamprev);
+    dfree(p->xg_part);
     if (p->chk_host) (void)hipHostFree(p->chk_host);
     if (p->side_stream) (void)hipStreamSynchronize(p->side_stream);
     for (int q = 0; q < ITCPD_MAX_ORDER; ++q) {
@@ -410,9 +413,26 @@ int itcpd_get_cpd(itcpd_plan* p, double* const* factors, double* lambda, int32_t
 int itcpd_run(itcpd_plan* p, itcpd_check* c, double* fit_trace) {
     if (!p || !c) return fail(ITCPD_E_ARG, "null argument");
     if (!p->have_cpd) return fail(ITCPD_E_STATE, "itcpd_run before itcpd_set_cpd / itcpd_random_cpd");
-    if (c->kind != ITCPD_CHECK_NONE && c->kind != ITCPD_CHECK_FIT)
+    if (c->kind != ITCPD_CHECK_NONE && c->kind != ITCPD_CHECK_FIT && c->kind != ITCPD_CHECK_CPDIFF &&
+        c->kind != ITCPD_CHECK_CPANGLE)
         return fail(ITCPD_E_ARG, "unknown check kind");
+    const bool cp_check = c->kind == ITCPD_CHECK_CPDIFF || c->kind == ITCPD_CHECK_CPANGLE;
+    if (cp_check && c->has_prev && !p->prev_valid)
+        return fail(ITCPD_E_STATE, "check holds a previous CPD that this plan does not have");
     DeviceGuard dg(p->device);
+    if (cp_check && !p->lamprev) {
+        // PrevCP storage and cross-Gram partials (row chunks bounded so that the
+        // partials stay within 2^24 doubles)
+        int rc;
+        for (int q = 0; q < p->N; ++q)
+            if ((rc = dmalloc(&p->Atprev[q], (size_t)(p->dims[q] * p->Rp)))) return rc;
+        if ((rc = dmalloc(&p->lamprev, (size_t)p->Rp))) return rc;
+        const int64_t RR = p->R * p->R;
+        int64_t ch = ((int64_t)1 << 24) / (p->N * RR);
+        ch = std::max<int64_t>(1, std::min<int64_t>(ch, 16));
+        p->xg_chunks = (int)ch;
+        if ((rc = dmalloc(&p->xg_part, (size_t)(p->N * ch * RR)))) return rc;
+    }
     // optimize.jl:8: the sweep loop runs while the local counter (starting at
     // check.iter) is below check.max_counter
     const int64_t nsweeps = (int64_t)c->max_counter - c->iter;
@@ -437,6 +457,12 @@ int itcpd_run(itcpd_plan* p, itcpd_check* c, double* fit_trace) {
     h->ref_norm = c->ref_norm;
     h->lastfit = c->lastfit;
     h->final_fit = c->final_fit;
+    if (cp_check) {
+        h->ref_norm = 0.0;
+        h->norm_prev = c->ref_norm;
+        h->has_prev = c->has_prev ? 1 : 0;
+    }
+    p->check_kind = c->kind;
     ITCPD_HIP(hipMemcpyAsync(p->chk, h, sizeof(DevCheck), hipMemcpyHostToDevice, p->stream));
     ITCPD_HIP(hipStreamSynchronize(p->stream));
 
@@ -446,7 +472,7 @@ int itcpd_run(itcpd_plan* p, itcpd_check* c, double* fit_trace) {
     // sweeps enqueued after convergence return at their first instruction (each
     // kernel tests chk->done).
     constexpr int64_t POLL_EVERY = 4;
-    const int64_t batch = (c->kind == ITCPD_CHECK_FIT) ? POLL_EVERY : nsweeps;
+    const int64_t batch = (c->kind != ITCPD_CHECK_NONE) ? POLL_EVERY : nsweeps;
     DevCheck* slots = p->chk_host + 1;
     int64_t polls_issued = 0, polls_seen = 0;
     int rc = ITCPD_OK;
@@ -455,7 +481,7 @@ int itcpd_run(itcpd_plan* p, itcpd_check* c, double* fit_trace) {
         for (int64_t b = 0; b < nb; ++b)
             if ((rc = sweep(p))) return rc;
         k += nb;
-        if (c->kind != ITCPD_CHECK_FIT || k >= nsweeps) continue;
+        if (c->kind == ITCPD_CHECK_NONE || k >= nsweeps) continue;
         const int slot = (int)(polls_issued % ITCPD_POLL_SLOTS);
         ITCPD_HIP(hipMemcpyAsync(&slots[slot], p->chk, sizeof(DevCheck), hipMemcpyDeviceToHost,
                                  p->stream));
@@ -478,7 +504,13 @@ int itcpd_run(itcpd_plan* p, itcpd_check* c, double* fit_trace) {
     }
     ITCPD_HIP(hipMemcpyAsync(h, p->chk, sizeof(DevCheck), hipMemcpyDeviceToHost, p->stream));
     ITCPD_HIP(hipStreamSynchronize(p->stream));
+    p->check_kind = 0;
     if (p->profiling && (rc = collect_timer(p, true))) return rc;
+    if (cp_check) {
+        c->ref_norm = h->norm_prev;
+        c->has_prev = h->has_prev;
+        p->prev_valid = h->has_prev != 0;
+    }
     c->iter = h->iter;
     c->counter = h->counter;
     c->total_iter = h->total_iter;
@@ -487,7 +519,7 @@ int itcpd_run(itcpd_plan* p, itcpd_check* c, double* fit_trace) {
     c->sweeps_run = h->sweeps > 0 ? h->sweeps : (int)std::min<int64_t>(nsweeps, INT32_MAX);
     c->converged = (h->done && !h->nan_flag) ? 1 : 0;
     if (c->kind == ITCPD_CHECK_NONE) c->sweeps_run = h->sweeps;
-    if (fit_trace && c->kind == ITCPD_CHECK_FIT && h->sweeps > 0)
+    if (fit_trace && c->kind != ITCPD_CHECK_NONE && h->sweeps > 0)
         ITCPD_HIP(hipMemcpy(fit_trace, p->fit_trace, (size_t)std::min(h->sweeps, p->trace_cap) * 8,
                             hipMemcpyDeviceToHost));
     if (h->nan_flag) return fail(ITCPD_E_NAN, "Error NAN");

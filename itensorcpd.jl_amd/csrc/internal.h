@@ -40,6 +40,9 @@ struct DevCheck {
     double ref_norm;
     double lastfit;
     double final_fit;
+    int has_prev;    // CPDiff / CPAngle: the previous sweep's CPD is held
+    int pad_;
+    double norm_prev;  // CPDiff: ||X_prev||^2, CPAngle: ||X_prev||
 };
 
 // One side of the dimension tree: the modes a GEMM contracts against T.
@@ -115,6 +118,14 @@ struct itcpd_plan {
     hipEvent_t ev_poll[ITCPD_POLL_SLOTS] = {nullptr};
     bool profiling = false;
     itcpd::KernelTimer timer;
+    // CPDiffCheck / CPAngleCheck: the previous sweep's CPD (rank-fastest factors and
+    // lambda), cross-Gram partials A_prev^T A per mode and row chunk
+    int check_kind = 0;
+    double* Atprev[ITCPD_MAX_ORDER] = {nullptr};
+    double* lamprev = nullptr;
+    double* xg_part = nullptr;
+    int xg_chunks = 0;
+    bool prev_valid = false;
 };
 
 namespace itcpd {
@@ -146,6 +157,7 @@ int fork_factor(itcpd_plan* p, int mode);  // factor of `mode` on the side strea
 int join_factor(itcpd_plan* p, int mode);  // main stream waits for it
 int launch_factor(itcpd_plan* p, int mode);
 int launch_factor_on(itcpd_plan* p, int mode, hipStream_t st);
+int launch_cp_check(itcpd_plan* p);
 int launch_solve(itcpd_plan* p, int mode, const double* Msrc, int nparts, int64_t stride,
                  bool finalize, bool fit_after);
 int launch_normalize(itcpd_plan* p, int mode, const double* Xsrc, bool fit_after);

@@ -688,6 +688,137 @@ __global__ __launch_bounds__(256) void k_fit(const double* __restrict__ inner_pa
     }
 }
 
+// ------------------------------------- CPDiffCheck / CPAngleCheck (CP vs CP)
+// The previous sweep's CPD against the current one: cp_cp_contract
+// (cp_contract.jl:32-52) is the Hadamard product over modes of the cross Grams
+// C_q = A_prev,q^T A_q, and <X_prev, X> = lambda_prev^T (.* C_q) lambda.
+struct FacPair {
+    const double* P[ITCPD_MAX_ORDER];   // previous factors, rank fastest (Rp x I_q)
+    const double* C[ITCPD_MAX_ORDER];   // current factors, rank fastest
+    int64_t I[ITCPD_MAX_ORDER];
+};
+
+// Cross-Gram partials over row chunks: block (x, chunk, q), thread e = 256 x + t
+// for the entry (r, r2) = (e / R, e % R) of C_q; a wave's threads share r (one
+// broadcast load of the previous factor per row) and read consecutive r2.
+__global__ __launch_bounds__(256) void k_cross_gram(FacPair fp, int R, int Rp, int chunks,
+                                                    double* __restrict__ part,
+                                                    const DevCheck* __restrict__ chk) {
+    if (chk->done || !chk->has_prev) return;
+    const int q = blockIdx.z, ch = blockIdx.y;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= R * R) return;
+    const int r = e / R, r2 = e % R;
+    const int64_t In = fp.I[q];
+    const int64_t per = (In + chunks - 1) / chunks;
+    const int64_t i0 = (int64_t)ch * per;
+    const int64_t i1 = i0 + per < In ? i0 + per : In;
+    const double* __restrict__ P = fp.P[q];
+    const double* __restrict__ Cc = fp.C[q];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t i = i0;
+    for (; i + 4 <= i1; i += 4) {
+        double x[4], y[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x[k] = P[(i + k) * Rp + r];
+            y[k] = Cc[(i + k) * Rp + r2];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a[k] += x[k] * y[k];
+    }
+    for (; i < i1; ++i) a[0] += P[i * Rp + r] * Cc[i * Rp + r2];
+    part[((int64_t)q * chunks + ch) * R * R + e] = (a[0] + a[1]) + (a[2] + a[3]);
+}
+
+// cp_diff_check.jl:20-71 / cp_angle_check.jl:20-73 after the sweep (one thread).
+static __device__ void cp_check_update(DevCheck* chk, double cross, double fact_square, double* trace,
+                                       int trace_cap) {
+    chk->iter += 1;
+    const int sweep = chk->sweeps;
+    chk->sweeps += 1;
+    const bool angle = chk->kind == ITCPD_CHECK_CPANGLE;
+    if (!chk->has_prev) {
+        // the first call stores PrevCP and its norm and returns false
+        chk->has_prev = 1;
+        chk->norm_prev = angle ? sqrt(fact_square) : fact_square;
+        if (trace && sweep < trace_cap) trace[sweep] = __builtin_nan("");
+        return;
+    }
+    double cur;
+    if (!angle) {
+        const double nres = sqrt(fabs(chk->norm_prev + fact_square - 2.0 * fabs(cross)));
+        cur = 1.0 - nres / sqrt(fabs(chk->norm_prev));
+        chk->norm_prev = fact_square;
+    } else {
+        const double ncur = sqrt(fact_square);
+        double th = cross / (ncur * chk->norm_prev);
+        th = th > 1.0 ? 1.0 : th;
+        cur = acos(th);
+        chk->norm_prev = ncur;
+    }
+    const double delta = fabs(chk->lastfit - cur);
+    chk->lastfit = cur;
+    if (trace && sweep < trace_cap) trace[sweep] = cur;
+    if (delta < chk->tolerance) {
+        chk->counter += 1;
+        if (chk->counter >= 2) {
+            chk->total_iter = chk->iter;
+            chk->iter = 0;
+            chk->counter = 0;
+            chk->final_fit = chk->lastfit;
+            chk->lastfit = 0.0;
+            chk->has_prev = 0;
+            chk->done = 1;
+            return;
+        }
+    } else {
+        chk->counter = 0;
+    }
+    if (chk->iter >= chk->max_counter) {
+        chk->total_iter = chk->iter;
+        chk->iter = 0;
+        chk->counter = 0;
+        chk->final_fit = chk->lastfit;
+        chk->lastfit = 0.0;
+        chk->has_prev = 0;
+    }
+}
+
+// ||X||^2 = lambda^T (.* G_q) lambda (norm_factors, converge_checks.jl:5-11) and,
+// with a previous CPD, <X_prev, X> from the cross-Gram partials (fixed order);
+// then the check's state update.
+__global__ __launch_bounds__(256) void k_cp_check(const double* __restrict__ part, int chunks, int R,
+                                                  GramSet gs, const double* __restrict__ lam,
+                                                  const double* __restrict__ lamprev,
+                                                  DevCheck* __restrict__ chk, double* __restrict__ trace,
+                                                  int trace_cap) {
+    if (chk->done) return;
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    const bool have = chk->has_prev != 0;
+    const int64_t RR = (int64_t)R * R;
+    double fs = 0.0, cr = 0.0;
+    for (int x = t; x < R * R; x += 256) {
+        const int r = x / R, r2 = x % R;
+        double h = 1.0;
+        for (int m = 0; m < gs.N; ++m) h *= gs.G[m][x];
+        fs += h * (lam[r] * lam[r2]);
+        if (have) {
+            double c = 1.0;
+            for (int m = 0; m < gs.N; ++m) {
+                double sacc = 0.0;
+                for (int ch = 0; ch < chunks; ++ch) sacc += part[((int64_t)m * chunks + ch) * RR + x];
+                c *= sacc;
+            }
+            cr += c * (lamprev[r] * lam[r2]);
+        }
+    }
+    const double fact_square = block_sum<256>(fs, red);
+    const double cross = block_sum<256>(cr, red);
+    if (t == 0) cp_check_update(chk, cross, fact_square, trace, trace_cap);
+}
+
 // ------------------------------------------------ row_norm + Gram partials
 // lambda from the per-block sums of squares; A = X ./ lambda (column-major,
 // Rp columns) and At (rank fastest); Gram partial over this block's rows; for
@@ -948,6 +1079,30 @@ int solve_blocks(int64_t In) {
 
 int launch_factor(itcpd_plan* p, int mode) { return launch_factor_on(p, mode, p->stream); }
 
+// CPDiffCheck / CPAngleCheck after the last mode of a sweep: cross Grams against
+// the held previous CPD, the check update, then PrevCP <- the current CPD.
+int launch_cp_check(itcpd_plan* p) {
+    const int R = (int)p->R, N = p->N;
+    FacPair fp{};
+    for (int q = 0; q < N; ++q) {
+        fp.P[q] = p->Atprev[q];
+        fp.C[q] = p->At[q];
+        fp.I[q] = p->dims[q];
+    }
+    const unsigned nx = (unsigned)cdiv((int64_t)R * R, 256);
+    hipLaunchKernelGGL(k_cross_gram, dim3(nx, (unsigned)p->xg_chunks, (unsigned)N), dim3(256), 0, p->stream,
+                       fp, R, (int)p->Rp, p->xg_chunks, p->xg_part, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_cp_check, dim3(1), dim3(256), 0, p->stream, p->xg_part, p->xg_chunks, R, grams_of(p),
+                       p->lam, p->lamprev, p->chk, p->fit_trace, p->trace_cap);
+    ITCPD_HIP(hipGetLastError());
+    for (int q = 0; q < N; ++q)
+        ITCPD_HIP(hipMemcpyAsync(p->Atprev[q], p->At[q], (size_t)(p->dims[q] * p->Rp) * 8,
+                                 hipMemcpyDeviceToDevice, p->stream));
+    ITCPD_HIP(hipMemcpyAsync(p->lamprev, p->lam, (size_t)p->Rp * 8, hipMemcpyDeviceToDevice, p->stream));
+    return ITCPD_OK;
+}
+
 int launch_factor_on(itcpd_plan* p, int mode, hipStream_t st) {
     const int R = (int)p->R;
     double* W = p->qr_aux;
@@ -1053,11 +1208,13 @@ int launch_solve(itcpd_plan* p, int mode, const double* Msrc, int nparts, int64_
     const int64_t In = p->dims[mode];
     const int nb = solve_nb(In, R);
     const int nbf = finalize_nb(In);
+    const bool cp_check = p->check_kind == ITCPD_CHECK_CPDIFF || p->check_kind == ITCPD_CHECK_CPANGLE;
     hipLaunchKernelGGL(k_finalize, dim3(nbf), dim3(256), (size_t)R * 8, p->stream, p->gram_part, nb,
                        p->X, p->Msum, In, R, (int)p->Rp, p->A[mode], p->At[mode], p->lam,
-                       p->G[mode], fit_after ? p->inner_part : (double*)nullptr, p->chk);
+                       p->G[mode], (fit_after && !cp_check) ? p->inner_part : (double*)nullptr, p->chk);
     ITCPD_HIP(hipGetLastError());
     if (!fit_after) return ITCPD_OK;
+    if (cp_check) return launch_cp_check(p);
     hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, p->stream, p->inner_part, nbf, R, grams_of(p),
                        p->lam, p->scal, p->chk, p->fit_trace, p->trace_cap);
     ITCPD_HIP(hipGetLastError());

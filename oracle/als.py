@@ -493,10 +493,16 @@ def optimize(T, cp: CPD, alg=None, check=None, verbose=False, trace: Optional[di
             st.fallbacks += int(fb)
             factors[n], lam = row_norm(X)
             st.gram_list[n] = factors[n].T @ factors[n]
+        store_only = isinstance(check, (CPDiffCheck, CPAngleCheck)) and check.PrevCP is None
         done = check_converge(check, factors, lam, st.gram_list, mtkrp, verbose)
         if isinstance(check, FitCheck):
             # this sweep's fit: after a reset (converged or max) it sits in final_fit
             st.fits.append(check.final_fit if check.iter == 0 else check.lastfit)
+        elif isinstance(check, (CPDiffCheck, CPAngleCheck)):
+            # NaN for the sweep that only stored PrevCP, else the sweep's value
+            attr = "lastfit" if isinstance(check, CPDiffCheck) else "lastangle"
+            st.fits.append(math.nan if store_only else
+                           (check.final_fit if check.iter == 0 else getattr(check, attr)))
         if done:
             break
         it += 1

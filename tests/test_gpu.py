@@ -130,6 +130,66 @@ def test_fitcheck_max_iterations_bookkeeping(itcpd):
     assert abs(cg.final_fit - cc.final_fit) < TOL
 
 
+@pytest.mark.parametrize("kind", ["diff", "angle"])
+def test_cp_vs_cp_checks_match_oracle(itcpd, kind):
+    # cp_diff_check.jl:20-71 / cp_angle_check.jl:20-73 on the device: per-sweep values
+    # (NaN for the sweep that only stores PrevCP), the stop, and the final state
+    T, _ = lowrank((40, 36, 32), 6, 1e-3, 21)
+    init = O.random_CPD(T, 6, np.random.default_rng(22))
+    G, Oc = ((itcpd.CPDiffCheck, O.CPDiffCheck) if kind == "diff"
+             else (itcpd.CPAngleCheck, O.CPAngleCheck))
+    cg, cc = G(1e-5, 60), Oc(1e-5, 60)
+    got, info = run_gpu(itcpd, T, init, cg)
+    tr = {}
+    ref = O.als_optimize(T, init, check=cc, trace=tr)
+    assert len(info["fits"]) == len(tr["fits"]) and info["converged"]
+    assert np.isnan(info["fits"][0]) and np.isnan(tr["fits"][0])
+    # both values come from cancelling sums: the diff value from |X_prev|^2 + |X|^2 -
+    # 2 <X_prev, X> (rounding ~1e-16 |X|^2 under a square root: ~1e-8), the angle from
+    # acos of a cosine near 1 (~sqrt(2e-16)); each side rounds on its own
+    atol = 1e-7
+    np.testing.assert_allclose(info["fits"][1:], tr["fits"][1:], rtol=0, atol=atol)
+    assert (cg.total_iter, cg.iter, cg.counter) == (cc.total_iter, cc.iter, cc.counter)
+    assert abs(cg.final_fit - cc.final_fit) < atol
+    assert not cg.has_prev and cc.PrevCP is None
+    assert factors_err(got, ref) < TOL
+
+
+@pytest.mark.parametrize("kind", ["diff", "angle"])
+def test_cp_vs_cp_checks_keep_prevcp_across_runs(itcpd, kind):
+    # max_counter = 1: the only call stores PrevCP (no max reset on that call), so the
+    # check keeps it; raising max_counter continues against the held CPD
+    T, _ = lowrank((20, 18, 16), 4, 1e-2, 23)
+    init = O.random_CPD(T, 4, np.random.default_rng(24))
+    G, Oc = ((itcpd.CPDiffCheck, O.CPDiffCheck) if kind == "diff"
+             else (itcpd.CPAngleCheck, O.CPAngleCheck))
+    cg, cc = G(0.0, 1), Oc(0.0, 1)
+    with itcpd.ALSPlan(T, 4) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        plan.run(cg)
+        assert cg.has_prev and cg.iter == 1
+        cg.max_counter = 5
+        info = plan.run(cg)
+        got = plan.get_cpd()
+    mid = O.als_optimize(T, init, check=cc)
+    assert cc.PrevCP is not None and cc.iter == 1
+    cc.max_counter = 5
+    tr = {}
+    ref = O.als_optimize(T, mid, check=cc, trace=tr)
+    atol = 1e-7   # cancelling sums, as in test_cp_vs_cp_checks_match_oracle
+    np.testing.assert_allclose(info["fits"], tr["fits"], rtol=0, atol=atol)
+    assert (cg.total_iter, cg.iter, cg.has_prev) == (cc.total_iter, cc.iter, cc.PrevCP is not None)
+    assert abs(cg.final_fit - cc.final_fit) < atol
+    assert factors_err(got, ref) < TOL
+    # a check that holds a previous CPD cannot start on a plan that does not
+    with itcpd.ALSPlan(T, 4) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        c2 = G(0.0, 5)
+        c2.has_prev = True
+        with pytest.raises(itcpd.ITCPDError):
+            plan.run(c2)
+
+
 def test_check_object_reused_across_calls(itcpd):
     # decompose.jl:53 reuses a check after resetting iter; the state carries over
     T, _ = lowrank((24, 20, 16), 4, 1e-3, 9)

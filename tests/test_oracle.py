@@ -177,6 +177,82 @@ def test_fitcheck_nan_raises():
         O.fit_state_update(c, float("nan"))
 
 
+def test_cp_cp_contract_is_the_dense_inner_product():
+    # cp_contract.jl:32-52: lambda1^T (.* A1_n^T A2_n) lambda2 = <X1, X2> of the dense tensors
+    rng = np.random.default_rng(31)
+    F1 = [rng.standard_normal((d, 4)) for d in (5, 6, 7)]
+    F2 = [rng.standard_normal((d, 3)) for d in (5, 6, 7)]
+    l1, l2 = rng.standard_normal(4), rng.standard_normal(3)
+    got = l1 @ O.cp_cp_contract(F1, F2) @ l2
+    want = float(np.sum(O.reconstruct(F1, l1) * O.reconstruct(F2, l2)))
+    assert abs(got - want) < 1e-12 * max(1.0, abs(want))
+
+
+def _cp(rng, dims, R):
+    F = [rng.standard_normal((d, R)) for d in dims]
+    F = [f / np.linalg.norm(f, axis=0) for f in F]
+    return F, rng.uniform(0.5, 2.0, R)
+
+
+def test_cpdiffcheck_values_are_explicit_relative_distances():
+    # cp_diff_check.jl:20-47: 1 - sqrt(|n_prev + n - 2 |<X_{n-1}, X_n>||) / sqrt(n_prev), the
+    # first call stores PrevCP only.  With a positive inner product this is the relative
+    # distance 1 - ||X_n - X_{n-1}|| / ||X_{n-1}||; with a negative one the reference's
+    # |inner| makes it 1 - ||X_n + X_{n-1}|| / ||X_{n-1}|| (kept as the reference has it)
+    rng = np.random.default_rng(32)
+    base, lb = _cp(rng, (6, 5, 4), 3)
+    cps = [([f + 0.05 * rng.standard_normal(f.shape) for f in base], lb) for _ in range(3)]
+    cps.append(([-base[0], base[1], base[2]], lb))       # the negated tensor: inner < 0
+    c = O.CPDiffCheck(1e-3, 10)
+    grams = lambda F: [f.T @ f for f in F]
+    assert O.check_converge(c, cps[0][0], cps[0][1], grams(cps[0][0])) is False
+    assert (c.iter, c.counter, c.lastfit) == (1, 0, 1.0) and c.PrevCP is not None
+    X = [O.reconstruct(F, l) for F, l in cps]
+    assert abs(c.norm_prev_iter - np.linalg.norm(X[0]) ** 2) < 1e-10 * np.linalg.norm(X[0]) ** 2
+    for k in (1, 2, 3):
+        O.check_converge(c, cps[k][0], cps[k][1], grams(cps[k][0]))
+        sign = 1.0 if np.sum(X[k] * X[k - 1]) > 0 else -1.0
+        want = 1 - np.linalg.norm(X[k] - sign * X[k - 1]) / np.linalg.norm(X[k - 1])
+        assert abs(c.lastfit - want) < 1e-10, k
+    assert c.iter == 4
+
+
+def test_cpanglecheck_values_are_explicit_angles():
+    # cp_angle_check.jl:20-49: acos(<X_n, X_{n-1}> / (||X_n|| ||X_{n-1}||)), clamped at 1
+    rng = np.random.default_rng(33)
+    c = O.CPAngleCheck(1e-3, 10)
+    cps = [_cp(rng, (6, 5, 4), 3) for _ in range(3)]
+    grams = lambda F: [f.T @ f for f in F]
+    O.check_converge(c, cps[0][0], cps[0][1], grams(cps[0][0]))
+    X = [O.reconstruct(F, l) for F, l in cps]
+    for k in (1, 2):
+        O.check_converge(c, cps[k][0], cps[k][1], grams(cps[k][0]))
+        cosv = np.sum(X[k] * X[k - 1]) / (np.linalg.norm(X[k]) * np.linalg.norm(X[k - 1]))
+        assert abs(c.lastangle - math.acos(min(cosv, 1.0))) < 1e-10
+    # the same CPD twice: angle 0 (the clamp keeps acos defined)
+    O.check_converge(c, cps[2][0], cps[2][1], grams(cps[2][0]))
+    assert c.lastangle < 1e-7
+
+
+def test_cp_checks_converge_and_reset_like_fitcheck():
+    # the two-small-deltas rule and the resets (PrevCP dropped on stop and on max)
+    rng = np.random.default_rng(34)
+    F, l = _cp(rng, (6, 5, 4), 3)
+    grams = [f.T @ f for f in F]
+    # the same CPD every sweep: diff values 1, 1 (deltas 0, 0 from lastfit = 1) stop at
+    # call 3; angle values 0, 0 (deltas 1, ~0, ~0 from lastangle = 1) stop at call 4
+    for cls, ncalls in ((O.CPDiffCheck, 3), (O.CPAngleCheck, 4)):
+        c = cls(1e-6, 100)
+        res = [O.check_converge(c, F, l, grams) for _ in range(ncalls)]
+        assert res == [False] * (ncalls - 1) + [True]
+        assert c.PrevCP is None and c.iter == 0 and c.total_iter == ncalls
+    for cls, attr in ((O.CPDiffCheck, "lastfit"), (O.CPAngleCheck, "lastangle")):
+        c = cls(0.0, 3)
+        res = [O.check_converge(c, F, l, grams) for _ in range(3)]
+        assert res == [False, False, False]
+        assert c.iter == 0 and c.total_iter == 3 and c.PrevCP is None and getattr(c, attr) == 0.0
+
+
 def test_nocheck_runs_exactly_max_sweeps():
     T, _ = lowrank((6, 5, 4), 2, 0.1, 14)
     cp = O.random_CPD(T, 2, np.random.default_rng(1))
