@@ -135,24 +135,22 @@ int alloc_factor_state(itcpd_plan* p) {
 
 // One ALS sweep (optimize.jl:17-30).  Mode n: V_n is factored, M_n computed, then
 // solve, row_norm, Gram refresh (and the fit after the last mode) run on the main
-// stream.  The first mode of each side gets M from a GEMM launch that fills every
-// CU, so its factorisation runs just before that GEMM on the main stream; the
-// other modes get M from a short Hadamard-reduce and their factorisation runs
-// beside it on the side stream.
+// stream.  The first mode of each side gets M from a GEMM launch; for R <= 64 its
+// factorisation runs inside that launch in one extra workgroup (otherwise right
+// after it).  The other modes get M from a short Hadamard-reduce and their
+// factorisation runs beside it on the side stream.
 int sweep(itcpd_plan* p) {
     int rc;
     const int N = p->N;
     MSrc ms{};
-    if ((rc = launch_factor(p, 0))) return rc;
-    if ((rc = launch_left_gemm(p, &ms))) return rc;
+    if ((rc = launch_left_gemm(p, &ms, 0))) return rc;   // with mode 0's factorisation
     for (int n = 0; n < p->s; ++n) {
         if (n > 0 && (rc = fork_factor(p, n))) return rc;
         if (p->s > 1 && (rc = launch_side_ttv(p, n, &ms))) return rc;
         if (n > 0 && (rc = join_factor(p, n))) return rc;
         if ((rc = launch_solve(p, n, ms.ptr, ms.nparts, ms.stride, true, n == N - 1))) return rc;
     }
-    if ((rc = launch_factor(p, p->s))) return rc;
-    if ((rc = launch_right_gemm(p, &ms))) return rc;
+    if ((rc = launch_right_gemm(p, &ms, p->s))) return rc;   // with mode s's factorisation
     for (int n = p->s; n < N; ++n) {
         if (n > p->s && (rc = fork_factor(p, n))) return rc;
         if (N - p->s > 1 && (rc = launch_side_ttv(p, n, &ms))) return rc;
@@ -201,6 +199,9 @@ int itcpd_plan_create(itcpd_plan** out, int32_t order, const int64_t* dims, int6
     if (!p) return fail(ITCPD_E_OOM, "host allocation failed");
     DeviceGuard dg(device);
     p->device = device;
+    if (hipDeviceGetAttribute(&p->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        p->num_cu < 1)
+        p->num_cu = 256;
     p->N = order;
     for (int q = 0; q < order; ++q) p->dims[q] = dims[q];
     p->R = rank;

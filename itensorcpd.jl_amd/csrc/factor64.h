@@ -1,0 +1,212 @@
+// The R <= 64 pivoted Cholesky + V^-1 (one 256-thread workgroup), shared by the
+// standalone factor kernel (solve.hip) and the GEMM launches that run it in an
+// extra workgroup (mttkrp.hip).
+#pragma once
+#include "dense_la.h"
+
+namespace itcpd {
+
+constexpr size_t FACTOR64_LDS = (size_t)(2 * 64 * 66) * 8;
+
+// Pivoted Cholesky for R <= 64 (dpstrf semantics as k_factor: absolute tolerance,
+// the first step only checks d > 0): pivot p is the remaining index with the
+// largest V(p,p) - sum_t U(t,p)^2 (first index on ties).  Right-looking on
+// registers: thread (w, l) holds rows 16w..16w+15 of column l of the Schur
+// complement; each step the wave owning row p hands it to all waves through LDS,
+// u_l = S(p, l) / U(j,j) for the remaining l, and every thread applies the rank-1
+// update to its 16 entries.  The pivot search runs redundantly in every wave on
+// identical data (the same u, the same running sums), so all waves agree without
+// communication.  UbT(l, j) = U(j, l) (zero for finished l) with ld 66.
+// Output in pivoted order: Uperm(j, t) = u^(j)_{piv[t]}, invdiag(j) = 1/U(j,j);
+// with Vinv, also V^-1 (R x R).  On rank deficiency all four waves build the QR
+// data.
+// Device function so that a GEMM launch can run it in one extra workgroup
+// (256 threads, FACTOR64_LDS bytes of dynamic LDS).
+__device__ __forceinline__ void factor64_dev(GramSet gs, int mode, int R, double tol,
+                                          double* __restrict__ Uperm, double* __restrict__ Vinv,
+                                          double* __restrict__ invdiag, int* __restrict__ piv,
+                                          int* __restrict__ finfo, double* __restrict__ W,
+                                          double* __restrict__ qtau, double* __restrict__ ztau,
+                                          int* __restrict__ jpvt, double* vwork) {
+    constexpr int LDT = 66;
+    extern __shared__ double S[];   // V (ld R), later U^-1 (ld LDT): 64 x LDT | UbT: 64 x LDT
+    __shared__ double diagU[64];
+    __shared__ double invdU[64];
+    __shared__ double rowb[2][64];
+    __shared__ int pivS[64];
+    __shared__ int s_rank;
+    __shared__ double redv[256];
+    __shared__ int redi[256];
+    double* UbT = S + 64 * LDT;
+    const int t = threadIdx.x, l = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    fill_hadamard<256>(S, gs, mode, R * R);
+    __syncthreads();
+    d16 sr;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int m = 16 * w + i;
+        sr[i] = (m < R && l < R) ? S[m * R + l] : 0.0;
+    }
+    bool rem = l < R;
+    const double vdiag = rem ? S[l * R + l] : 0.0;
+    double sumsq = 0.0;
+    uint64_t remmask = R >= 64 ? ~0ull : ((1ull << R) - 1);
+    int rank = R;
+    // Software pipelined: at the top of step j the registers hold the Schur
+    // complement with the updates of steps < j-1 applied; step j-1's rank-1 update
+    // (this wave's 16 entries of row p_{j-1} in rw, coefficient cu = u_l /
+    // U(j-1,j-1)) is pending.  The owner of row p_j corrects just that entry before
+    // handing the row over, and every wave applies the pending update after the
+    // barrier, while its LDS reads are in flight.  The pivot search needs only the
+    // running sums.
+    // One step; rc holds the pending row values, the new ones go to rn (the loop
+    // below alternates two register sets instead of copying).  False when the
+    // pivot fails the tolerance test (rank = j).
+    double cu = 0.0;
+    auto step = [&](int j, const d16& rc, d16& rn) -> bool {
+        double d;
+        const int p = wave_first_max(rem ? vdiag - sumsq : -__builtin_inf(), d);
+        const bool bad = (p >= R) || !((remmask >> p) & 1) || ((j == 0) ? !(d > 0.0) : !(d > tol));
+        if (bad) { rank = j; return false; }
+        if (w == (p >> 4)) rowb[j & 1][l] = fma(-rc[p & 15], cu, sr[p & 15]);
+        remmask &= ~(1ull << p);
+        if (l == p) rem = false;
+        __syncthreads();
+        // rows of finished indices take the update too (never read again), so the
+        // 16 row values are plain loads with no per-row condition
+        const double* rb = rowb[j & 1];
+        const double rl = rb[l];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) rn[i] = rb[16 * w + i];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sr[i] = fma(-rc[i], cu, sr[i]);
+        // 1/sqrt(d) from the hardware estimate (relative error < 2^-22) and two
+        // Newton steps y <- y + y (1/2 - d y^2 / 2); U(j,j) = d / sqrt(d).  d > 0
+        // here; the clamp keeps the estimate finite below 2^-1020.
+        const double dc = fmax(d, 0x1p-1020);
+        const double h = 0.5 * dc;
+        double y = __builtin_amdgcn_rsq(dc);
+        y = fma(y, fma(-h * y, y, 0.5), y);
+        y = fma(y, fma(-h * y, y, 0.5), y);
+        const double inv = y, ujj = dc * y;
+        const double u = rem ? rl * inv : 0.0;
+        sumsq = fma(u, u, sumsq);
+        cu = u * inv;
+        if (t == 0) { diagU[j] = ujj; invdU[j] = inv; pivS[j] = p; }
+        if (w == 0 && l < R) UbT[l * LDT + j] = u;
+        return true;
+    };
+    d16 rwa, rwb;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) { rwa[i] = 0.0; rwb[i] = 0.0; }
+    for (int j = 0; j < R; j += 2) {
+        if (!step(j, rwa, rwb)) break;
+        if (j + 1 >= R || !step(j + 1, rwb, rwa)) break;
+    }
+    if (t == 0) s_rank = rank;
+    __syncthreads();
+    if (s_rank == R) {
+        for (int j = t; j < R; j += 256) {
+            invdiag[j] = invdU[j];
+            piv[j] = pivS[j];
+        }
+        if (!Vinv) {
+            for (int x = t; x < R * R; x += 256) {
+                const int j = x / R, tt = x % R;
+                double v = 0.0;
+                if (tt > j) v = UbT[pivS[tt] * LDT + j];
+                else if (tt == j) v = diagU[j];
+                Uperm[x] = v;
+            }
+        } else {
+            // V^-1 = P U^-1 U^-T P^T for the MFMA row solves.  U in pivoted order,
+            // Up(i, k) = U(i, piv k) = UbT(piv k, i) for i < k < R (identity beyond
+            // R), is read in place; U^-1 goes to Ui (the S region, ld LDT) by 16 x 16
+            // blocks: diagonal blocks by substitution (wave b, lane c: column c of
+            // block b), then X_ij = -X_ii sum_{k=i+1..j} Up_ik X_kj by block
+            // diagonals on MFMA; finally the upper tiles of Z = U^-1 U^-T on MFMA
+            // (Z(a, b) sums over k >= max(a, b) only) and Vinv(piv a, piv b) =
+            // Vinv(piv b, piv a) = Z(a, b).  f64 16x16x4 MFMA layout: A lane (c, g)
+            // = A[c][k0+g], B lane = B[k0+g][c], result v = D[g+4v][c]; the result of
+            // one product is the B operand of the next one as it stands (k0 = 4v).
+            const int c = l & 15, g = l >> 4;
+            double* Ui = S;
+            if (t >= R && t < 64) invdU[t] = 1.0;
+            for (int x = t; x < 64 * 64; x += 256) {
+                const int i = x >> 6, k = x & 63;
+                if ((i >> 4) > (k >> 4)) Ui[i * LDT + k] = 0.0;
+            }
+            __syncthreads();
+            if (l < 16) {
+                const int b0 = 16 * w;
+                int pk[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) pk[k] = (b0 + k < R) ? pivS[b0 + k] * LDT : -1;
+                double xv[16];
+#pragma unroll
+                for (int i = 15; i >= 0; --i) {
+                    double acc0 = (i == l) ? 1.0 : 0.0, acc1 = 0.0;
+#pragma unroll
+                    for (int k = i + 1; k < 16; ++k) {
+                        const double uik = pk[k] >= 0 ? UbT[pk[k] + b0 + i] : 0.0;
+                        if (k & 1) acc1 -= uik * xv[k];
+                        else acc0 -= uik * xv[k];
+                    }
+                    xv[i] = (i <= l) ? (acc0 + acc1) * invdU[b0 + i] : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < 16; ++i) Ui[(b0 + i) * LDT + b0 + l] = xv[i];
+            }
+            __syncthreads();
+            for (int dd = 1; dd < 4; ++dd) {
+                const int bi = w, bj = w + dd;
+                if (bj < 4) {
+                    const int i = 16 * bi + c;
+                    d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+                    for (int bk = bi + 1; bk <= bj; ++bk)
+#pragma unroll
+                        for (int k0 = 0; k0 < 16; k0 += 4) {
+                            const int kk = 16 * bk + k0 + g;
+                            const double a = (kk < R && i < R) ? UbT[pivS[kk] * LDT + i] : 0.0;
+                            acc = mfma_f64(a, Ui[kk * LDT + 16 * bj + c], acc);
+                        }
+                    d4 out = (d4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        out = mfma_f64(-Ui[(16 * bi + c) * LDT + 16 * bi + 4 * v + g], acc[v], out);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) Ui[(16 * bi + g + 4 * v) * LDT + 16 * bj + c] = out[v];
+                }
+                __syncthreads();
+            }
+            // upper tiles (ta <= tb) of Z, 20 k-steps per wave
+            constexpr int ZT[4][3][2] = {{{0, 0}, {0, 3}, {-1, -1}},
+                                         {{0, 1}, {1, 3}, {2, 3}},
+                                         {{1, 1}, {0, 2}, {-1, -1}},
+                                         {{1, 2}, {2, 2}, {3, 3}}};
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int ta = ZT[w][q][0], tb = ZT[w][q][1];
+                if (ta < 0 || 16 * tb >= R) continue;
+                d4 acc = (d4){0.0, 0.0, 0.0, 0.0};
+                for (int k0 = 16 * tb; k0 < 64; k0 += 4)
+                    acc = mfma_f64(Ui[(16 * ta + c) * LDT + k0 + g], Ui[(16 * tb + c) * LDT + k0 + g], acc);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int ra = 16 * ta + g + 4 * v, rb = 16 * tb + c;
+                    if (ra < R && rb < R) {
+                        Vinv[pivS[ra] * R + pivS[rb]] = acc[v];
+                        if (ta != tb) Vinv[pivS[rb] * R + pivS[ra]] = acc[v];
+                    }
+                }
+            }
+        }
+        if (t == 0) { finfo[0] = R; finfo[1] = 0; finfo[2] = R; }
+        return;
+    }
+    qr_fallback<256>(gs, mode, R, W, qtau, ztau, jpvt, vwork, finfo, redv, redi, s_rank);
+}
+
+
+}  // namespace itcpd

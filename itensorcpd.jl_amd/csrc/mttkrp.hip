@@ -11,6 +11,7 @@
 // from its partial by a Hadamard-reduce against the side's other (current)
 // factors.  T is read twice per sweep.
 #include "internal.h"
+#include "factor64.h"
 
 namespace itcpd {
 
@@ -158,6 +159,24 @@ __global__ __launch_bounds__(256, 2) void k_tk(const TT* __restrict__ T, int64_t
         }
 }
 
+// A mode's R <= 64 factorisation riding along with a GEMM launch: block x = 0 of
+// the grid runs factor64_dev (block (0, 0, 0)) or exits; the GEMM tiles use
+// blockIdx.x - 1.  The factorisation reads only Grams, which no GEMM touches.
+struct FactorJob {
+    GramSet gs;
+    int on, mode, R;
+    double tol;
+    double *Uperm, *Vinv, *invdiag, *W, *qtau, *ztau, *vwork;
+    int *piv, *finfo, *jpvt;
+};
+__device__ __forceinline__ bool factor_block(const FactorJob& fj) {
+    if (!fj.on || blockIdx.x != 0) return false;
+    if (blockIdx.y == 0 && blockIdx.z == 0)
+        factor64_dev(fj.gs, fj.mode, fj.R, fj.tol, fj.Uperm, fj.Vinv, fj.invdiag, fj.piv, fj.finfo, fj.W,
+                     fj.qtau, fj.ztau, fj.jpvt, fj.vwork);
+    return true;
+}
+
 // ------------------------------------------------- TK GEMM, LDS ring form
 // The same product for FP64 T, 64-rank blocks and 256-row workgroup tiles, with
 // T and the KRP rows staged through a ring of LDS slots filled by direct
@@ -191,15 +210,17 @@ __global__ __launch_bounds__(256, 2) void k_tk_ring(const double* __restrict__ T
                                                     int64_t nsteps_i, int64_t kk_per_split,
                                                     int64_t kk_total, int Rp, int R,
                                                     double* __restrict__ out, int64_t out_ld,
-                                                    int64_t split_stride,
+                                                    int64_t split_stride, FactorJob fj,
                                                     const DevCheck* __restrict__ chk) {
     constexpr int RB = 4, MB = 4, U = 2;
     if (chk->done) return;
+    if (factor_block(fj)) return;
+    const int64_t bx = (int64_t)blockIdx.x - (fj.on ? 1 : 0);
     extern __shared__ __attribute__((aligned(16))) unsigned char ring[];
     const int t = threadIdx.x, lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int c = lane & 15, g = lane >> 4;
-    const int64_t mwg = (int64_t)blockIdx.x * 256;
+    const int64_t mwg = bx * 256;
     const int r0 = blockIdx.y * 64;
     const int64_t kk0 = (int64_t)blockIdx.z * kk_per_split;
     const int64_t kk1 = kk0 + kk_per_split < kk_total ? kk0 + kk_per_split : kk_total;
@@ -1039,16 +1060,18 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void k_tr_ring(const double* __res
                                                              const double* __restrict__ At1, int Rp,
                                                              int64_t kk_per, int64_t kk_total, int R,
                                                              double* __restrict__ out, int64_t split_stride,
-                                                             const DevCheck* __restrict__ chk) {
+                                                             FactorJob fj, const DevCheck* __restrict__ chk) {
     constexpr int RPW = 64 / NW;      // runs of T (and of A0) loaded per wave
     constexpr int NCT = 16 / NW;      // 16-column tiles per wave
     if (chk->done) return;
+    if (NW == 4 && factor_block(fj)) return;
+    const int64_t bx = (int64_t)blockIdx.x - (fj.on ? 1 : 0);
     extern __shared__ __attribute__((aligned(16))) unsigned char ring[];
     const int t = threadIdx.x, lane = t & 63;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int c = lane & 15, g = lane >> 4;
     const int rb = wave & 3, ch = wave >> 2;
-    const int64_t n0 = (int64_t)blockIdx.x * 64;
+    const int64_t n0 = bx * 64;
     const int r0 = blockIdx.y * 64;
     const int64_t nstg = I0 / 32;
     const int64_t q0 = (int64_t)blockIdx.z * kk_per;
@@ -1175,6 +1198,34 @@ int64_t pick_split(int64_t base_wgs, int64_t kk_total, int64_t target, int64_t m
     return nz;
 }
 
+// The factor job of the plan's pending fused factorisation (off when none).
+FactorJob factor_job(itcpd_plan* p) {
+    FactorJob fj{};
+    fj.on = 0;
+    // only along a long GEMM (>= 8 GFLOP, i.e. >= ~150 us): the factorisation's
+    // steps run slower beside GEMM workgroups and must stay hidden under them
+    if (p->fuse_factor_mode < 0 || p->R > 64 || 2.0 * (double)p->ML * (double)p->MR * (double)p->R < 8e9)
+        return fj;
+    const int R = (int)p->R;
+    fj.on = 1;
+    fj.mode = p->fuse_factor_mode;
+    fj.R = R;
+    fj.tol = 1e-6;
+    fj.gs.N = p->N;
+    for (int m = 0; m < p->N; ++m) fj.gs.G[m] = p->G[m];
+    fj.Uperm = p->U;
+    fj.Vinv = p->Upad;
+    fj.invdiag = p->invdiag;
+    fj.piv = p->piv;
+    fj.finfo = p->finfo;
+    fj.W = p->qr_aux;
+    fj.qtau = fj.W + (size_t)R * R;
+    fj.ztau = fj.qtau + R;
+    fj.vwork = fj.ztau + R;
+    fj.jpvt = p->piv + R;
+    return fj;
+}
+
 // The LDS ring form of the left GEMM applies: FP64 T, 256-row tiles, 64-rank
 // blocks, stages of 8 indices of the first right mode.
 bool tk_ring_ok(const itcpd_plan* p, const Side& sd) {
@@ -1186,11 +1237,14 @@ int run_tk(itcpd_plan* p, const Side& sd, double* out, int64_t out_ld, int64_t n
            int64_t kk_per, int64_t kk_total, int64_t nsteps_i, int64_t split_stride) {
     constexpr int U = 2;
     if (sizeof(TT) == 8 && RB == 4 && tk_ring_ok(p, sd) && kk_per % 2 == 0) {
-        dim3 grid((unsigned)(p->ML / 256), (unsigned)(p->Rp / 64), (unsigned)nz);
-        hipLaunchKernelGGL(k_tk_ring, grid, dim3(256), TKR_LDS, p->stream, (const double*)p->T, p->ML, sd,
+        const FactorJob fj = factor_job(p);
+        dim3 grid((unsigned)(p->ML / 256 + fj.on), (unsigned)(p->Rp / 64), (unsigned)nz);
+        const size_t lds = fj.on ? std::max<size_t>(TKR_LDS, FACTOR64_LDS) : (size_t)TKR_LDS;
+        hipLaunchKernelGGL(k_tk_ring, grid, dim3(256), lds, p->stream, (const double*)p->T, p->ML, sd,
                            nsteps_i, kk_per, kk_total, (int)p->Rp, (int)p->R, out, out_ld, split_stride,
-                           p->chk);
+                           fj, p->chk);
         ITCPD_HIP(hipGetLastError());
+        p->fused_factor = fj.on != 0;
         return ITCPD_OK;
     }
     dim3 grid((unsigned)cdiv(p->ML, 64 * MB), (unsigned)(p->Rp / (16 * RB)), (unsigned)nz);
@@ -1348,7 +1402,29 @@ int finish_parts(itcpd_plan* p, double* parts, int64_t nz, int64_t n, double* ds
 int launch_left_gemm_lp(itcpd_plan* p, MSrc* ms);
 int launch_right_gemm_lp(itcpd_plan* p, MSrc* ms);
 
-int launch_left_gemm(itcpd_plan* p, MSrc* ms) {
+int launch_left_gemm_impl(itcpd_plan* p, MSrc* ms);
+int launch_right_gemm_impl(itcpd_plan* p, MSrc* ms);
+
+// A GEMM launch, optionally carrying a factorisation (see FactorJob); without a
+// fused workgroup the factorisation is launched right after it on the same stream.
+int launch_gemm_with_factor(itcpd_plan* p, MSrc* ms, bool left, int factor_mode, bool* fused) {
+    p->fuse_factor_mode = (p->prec == ITCPD_FP64 && p->R <= 64) ? factor_mode : -1;
+    p->fused_factor = false;
+    const int rc = left ? launch_left_gemm_impl(p, ms) : launch_right_gemm_impl(p, ms);
+    p->fuse_factor_mode = -1;
+    if (fused) *fused = p->fused_factor;
+    if (rc) return rc;
+    if (factor_mode >= 0 && !p->fused_factor) return launch_factor(p, factor_mode);
+    return ITCPD_OK;
+}
+int launch_left_gemm(itcpd_plan* p, MSrc* ms, int factor_mode, bool* fused) {
+    return launch_gemm_with_factor(p, ms, true, factor_mode, fused);
+}
+int launch_right_gemm(itcpd_plan* p, MSrc* ms, int factor_mode, bool* fused) {
+    return launch_gemm_with_factor(p, ms, false, factor_mode, fused);
+}
+
+int launch_left_gemm_impl(itcpd_plan* p, MSrc* ms) {
     if (p->prec != ITCPD_FP64) return launch_left_gemm_lp(p, ms);
     const Side sd = make_side(p, p->s, p->N - p->s, /*rank_fast=*/true);
     const int MB = mb_of(p->RB);
@@ -1379,7 +1455,7 @@ int launch_left_gemm(itcpd_plan* p, MSrc* ms) {
 
 // Right GEMM: rows = right multi-index (MR), reduction = left side.  Produces PR
 // (N-s > 1) or the M of mode N-1 (N-s == 1).
-int launch_right_gemm(itcpd_plan* p, MSrc* ms) {
+int launch_right_gemm_impl(itcpd_plan* p, MSrc* ms) {
     if (p->prec != ITCPD_FP64) return launch_right_gemm_lp(p, ms);
     const Side sd = make_side(p, 0, p->s, /*rank_fast=*/false);
     if (p->s <= 2 && p->elt_bytes == 8 && sd.dim[0] % 32 == 0 && p->MR % 64 == 0 && p->Rp % 64 == 0) {
@@ -1389,7 +1465,13 @@ int launch_right_gemm(itcpd_plan* p, MSrc* ms) {
         const int gy = (int)(p->Rp / 64);
         const int64_t kk_total = (sd.dim[0] / 32) * (p->ML / sd.dim[0]);
         const int64_t out_elems = p->MR * p->R;
-        int64_t nz = pick_split(ntx * gy, kk_total, 256 * TRR_WG_PER_CU, 8);
+        // The right GEMM does not take a factorisation along: beside its
+        // workgroups (LDS-heavy, one round for the whole launch) the 64 dependent
+        // steps of the factorisation slowed down until they set the kernel's end
+        // (512^3 rank 64: 446 us instead of 320 us).
+        const FactorJob fj{};
+        const int64_t slots = (int64_t)p->num_cu * TRR_WG_PER_CU;   // workgroups resident at once
+        int64_t nz = pick_split(ntx * gy, kk_total, slots, 8);
         if (nz > 1 && (size_t)(nz * out_elems) > p->parts_elems) nz = (int64_t)(p->parts_elems / out_elems);
         if (nz < 1) nz = 1;
         const int64_t kk_per = cdiv(kk_total, nz);
@@ -1399,11 +1481,13 @@ int launch_right_gemm(itcpd_plan* p, MSrc* ms) {
         {
             Timed tm(p, 1, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
                      (double)p->ML * (double)p->MR * p->elt_bytes);
-            hipLaunchKernelGGL((k_tr_ring<TRR_NW, TRR_NS>), dim3((unsigned)ntx, (unsigned)gy, (unsigned)nz),
-                               dim3(TRR_NW * 64), TRR_LDS,
+            const size_t lds = fj.on ? std::max<size_t>(TRR_LDS, FACTOR64_LDS) : (size_t)TRR_LDS;
+            hipLaunchKernelGGL((k_tr_ring<TRR_NW, TRR_NS>), dim3((unsigned)(ntx + fj.on), (unsigned)gy, (unsigned)nz),
+                               dim3(TRR_NW * 64), lds,
                                p->stream, (const double*)p->T, p->ML, p->MR, sd.dim[0],
                                (const double*)sd.fac[0], p->s == 2 ? (const double*)p->At[1] : nullptr,
-                               (int)p->Rp, kk_per, kk_total, (int)p->R, out, out_elems, p->chk);
+                               (int)p->Rp, kk_per, kk_total, (int)p->R, out, out_elems, fj, p->chk);
+            p->fused_factor = fj.on != 0;
             ITCPD_HIP(hipGetLastError());
         }
         return finish_parts(p, p->parts, nz, out_elems, dst, ms, p->N - p->s == 1);
