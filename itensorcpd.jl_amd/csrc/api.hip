@@ -103,6 +103,28 @@ hipError_t create_streams(itcpd_plan* p) {
     return hipStreamCreateWithFlags(&p->side_stream, hipStreamNonBlocking);
 }
 
+// Buffers of the reduced-precision GEMMs, allocated on the first switch away from
+// FP64: the converted first factor; for an FP32-stored tensor also a side's
+// Khatri-Rao product (rank padded to 128, 4 bytes per element) and room for 8
+// split-K partials of the larger side's output.
+int ensure_lp_buffers(itcpd_plan* p) {
+    int rc;
+    if (!p->Af && (rc = dmalloc(&p->Af, (size_t)(p->maxI * p->R)))) return rc;
+    if (p->elt_bytes != 4) return ITCPD_OK;
+    const int64_t Mmax = std::max(p->ML, p->MR);
+    if (!p->Kt && (rc = dmalloc(&p->Kt, (size_t)(Mmax * cdiv(p->R, 128) * 128)))) return rc;
+    const size_t want = std::min<size_t>((size_t)8 * (size_t)(Mmax * p->R), (size_t)1 << 27);
+    if (p->parts_elems < want) {
+        ITCPD_HIP(hipStreamSynchronize(p->stream));
+        dfree(p->parts);
+        p->parts = nullptr;
+        p->parts_elems = 0;
+        if ((rc = dmalloc(&p->parts, want))) return rc;
+        p->parts_elems = want;
+    }
+    return ITCPD_OK;
+}
+
 // Per-mode factor state shared by plans and the plan-free random init.
 int alloc_factor_state(itcpd_plan* p) {
     int rc;
@@ -258,7 +280,6 @@ int itcpd_plan_create(itcpd_plan** out, int32_t order, const int64_t* dims, int6
     if ((rc = dmalloc(&p->qr_aux, (size_t)(R * R + 4 * R)))) return bail(rc);
     if ((rc = dmalloc(&p->piv, (size_t)(2 * R)))) return bail(rc);
     if ((rc = dmalloc(&p->finfo, 4))) return bail(rc);
-    if (prec != ITCPD_FP64 && (rc = dmalloc(&p->Af, (size_t)(maxI * R)))) return bail(rc);
     if (p->s > 1 && (rc = dmalloc(&p->PL, (size_t)(p->ML * R)))) return bail(rc);
     if (order - p->s > 1 && (rc = dmalloc(&p->PR, (size_t)(p->MR * R)))) return bail(rc);
     {
@@ -268,6 +289,7 @@ int itcpd_plan_create(itcpd_plan** out, int32_t order, const int64_t* dims, int6
         p->parts_elems = want;
         if ((rc = dmalloc(&p->parts, want))) return bail(rc);
     }
+    if (prec != ITCPD_FP64 && (rc = ensure_lp_buffers(p))) return bail(rc);
     if ((rc = dmalloc(&p->chk, 1))) return bail(rc);
     e = hipHostMalloc((void**)&p->chk_host, sizeof(DevCheck) * (1 + ITCPD_POLL_SLOTS),
                       hipHostMallocDefault);
@@ -305,6 +327,7 @@ int itcpd_plan_destroy(itcpd_plan* p) {
     dfree(p->piv);
     dfree(p->finfo);
     dfree(p->Af);
+    dfree(p->Kt);
     dfree(p->Upad);
     dfree(p->PL);
     dfree(p->PR);
@@ -623,9 +646,9 @@ int itcpd_plan_set_precision(itcpd_plan* p, int32_t prec) {
         return fail(ITCPD_E_UNSUPPORTED, "prec must be ITCPD_FP64, ITCPD_FP32 or ITCPD_BF16X3");
     DeviceGuard dg(p->device);
     ITCPD_HIP(hipStreamSynchronize(p->stream));
-    if (prec != ITCPD_FP64 && !p->Af) {
+    if (prec != ITCPD_FP64) {
         int rc;
-        if ((rc = dmalloc(&p->Af, (size_t)(p->maxI * p->R)))) return rc;
+        if ((rc = ensure_lp_buffers(p))) return rc;
     }
     p->prec = prec;
     return ITCPD_OK;

@@ -105,6 +105,7 @@ struct itcpd_plan {
     int* finfo = nullptr;                      // [0] = rank, [1] = used fallback, [2] = QR rank
     double* PL = nullptr;                      // ML x Rp partial of the left side (if s > 1)
     float* Af = nullptr;                       // FP32 copy of the KRP side's first factor (reduced precision)
+    float* Kt = nullptr;                       // FP32-stored T, reduced precision: a side's KRP, rank-major
     double* PR = nullptr;                      // MR x Rp partial of the right side (if N-s > 1)
     double* parts = nullptr;                   // split-K partial buffer
     size_t parts_elems = 0;

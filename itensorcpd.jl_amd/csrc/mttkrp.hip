@@ -842,6 +842,213 @@ __global__ __launch_bounds__(256, 2) void k_tr_lp(const TT* __restrict__ T, int6
         }
 }
 
+// ------------------------------------ reduced-precision GEMMs, LDS ring form
+// For an FP32-stored tensor whose GEMM dimensions fit 256-row tiles and 32-index
+// stages.  The side's Khatri-Rao product is formed once per GEMM by k_krp_lp:
+// Kt(r, m) = prod_q A_q(digit_q(m), r) for m over the reduction side and r over
+// the rank padded to 128, rank-major (m fastest), FP32 or BF16 hi followed by
+// BF16 lo.  The GEMM is then plain:
+//   left:  out(mL, r) = sum_mR T(mL, mR) Kt(r, mR)   (tile: 256 rows mL)
+//   right: out(mR, r) = sum_mL T(mL, mR) Kt(r, mL)   (tile: 256 columns mR)
+// A workgroup of 8 waves covers 256 rows / columns x 128 ranks; wave w owns the
+// 64 rows / columns 64 (w & 3) .. and the 64 ranks 64 (w >> 2) ..: 4 x 4 MFMA
+// tiles.  A stage is 32 consecutive reduction indices: the T tile (32 KB) and the
+// Kt tile (128 ranks x 128 bytes) arrive in an LDS slot by direct global->LDS
+// loads (lane l of a wave writes 16 bytes at slot offset + 16 l); stages q+1 and
+// q+2 are in flight while stage q is computed.
+//
+// LDS layouts.  Rows of 128 bytes (a Kt rank row, or 32 indices of one T column
+// in the right GEMM) are stored in pairs, 256-byte lines of 16 chunks: chunk q of
+// row j sits at line j >> 1, position (8 (j & 1) + q) ^ ((j >> 1) & 15), so the
+// 16 lanes that read chunk q of 16 consecutive rows hit 16 distinct bank groups.
+// The left GEMM's T tile is 32 columns of 256 rows (1 KB each, unswizzled: 16
+// lanes read 256 contiguous bytes).
+//
+// Lane (c, g) takes reduction indices 8g .. 8g+7 of a stage: for BF16x3 the 8
+// k-values of one 16x16x32 MFMA, for FP32 eight 16x16x4 MFMAs (MFMA e takes
+// k = 8g + e).  The right GEMM reads two 16-byte chunks of each of its rows; the
+// left GEMM reads rows 4c .. 4c+3 of columns 8g .. 8g+7 (one 16-byte read per
+// column), so its MFMA column c of tile b is row 4c + b.
+constexpr int LPR_NS = 3;
+constexpr int LPR_TBYTES = 32 * 1024;
+constexpr int LPR_KBYTES = 128 * 128;
+constexpr int LPR_SLOT = LPR_TBYTES + LPR_KBYTES;
+constexpr int LPR_LDS = LPR_NS * LPR_SLOT;
+
+// byte offset of chunk q (16 bytes) of 128-byte row j in a paired-line region
+__device__ __forceinline__ int lpr_chunk(int j, int q) {
+    return (j >> 1) * 256 + 16 * ((((j & 1) << 3) | q) ^ ((j >> 1) & 15));
+}
+
+template <bool BF>
+__global__ __launch_bounds__(256) void k_krp_lp(Side sd, int64_t M, int R, int Rk,
+                                                void* __restrict__ Kt,
+                                                const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= M * Rk) return;
+    const int r = (int)(idx / M);
+    const int64_t m = idx - (int64_t)r * M;
+    double w = 0.0;
+    if (r < R) {
+        w = 1.0;
+        int64_t t = m;
+        for (int q = 0; q < sd.nm; ++q) {
+            const int64_t d = t % sd.dim[q];
+            t /= sd.dim[q];
+            w *= sd.fac[q][d + sd.dim[q] * r];
+        }
+    }
+    const float x = (float)w;
+    if constexpr (BF) {
+        __bf16* o = reinterpret_cast<__bf16*>(Kt);
+        const __bf16 hi = (__bf16)x;
+        o[idx] = hi;
+        o[M * Rk + idx] = (__bf16)(x - (float)hi);
+    } else {
+        reinterpret_cast<float*>(Kt)[idx] = x;
+    }
+}
+
+template <bool LEFT, bool BF>
+__global__ __launch_bounds__(512, 1) void k_lp_ring(const float* __restrict__ T, int64_t ML,
+                                                    const void* __restrict__ Kt, int64_t Mk,
+                                                    int64_t kt_lo_off, int64_t kper, int64_t nstages,
+                                                    int R, double* __restrict__ out, int64_t out_ld,
+                                                    int64_t split_stride,
+                                                    const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char ring[];
+    const int t = threadIdx.x, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int c = lane & 15, g = lane >> 4;
+    const int wq = wave & 3, wr = wave >> 2;
+    const int64_t n0 = (int64_t)blockIdx.x * 256;
+    const int r0 = blockIdx.y * 128;
+    const int64_t s0 = (int64_t)blockIdx.z * kper;
+    const int64_t s1 = s0 + kper < nstages ? s0 + kper : nstages;
+    const int nst = (int)(s1 - s0);
+
+    // lane's (row, chunk) in a 1 KB paired-line block: line 4 ins + (l >> 4), position l & 15
+    const int lline = lane >> 4, lpos = lane & 15;
+    // stage q into slot q % LPR_NS: wave w loads 4 KB of T and 2 KB of Kt
+    auto issue = [&](int q) {
+        const int64_t k0 = (s0 + q) * 32;
+        unsigned char* slot = ring + (q % LPR_NS) * LPR_SLOT;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int ins = 4 * wave + u;
+            if constexpr (LEFT) {
+                lds_dma16(T + (k0 + ins) * ML + n0 + 4 * lane, slot + ins * 1024);
+            } else {
+                const int L = 4 * ins + lline, P = lpos ^ (L & 15);
+                const int j = 2 * L + (P >> 3), qc = P & 7;
+                lds_dma16(T + (n0 + j) * ML + k0 + 4 * qc, slot + ins * 1024);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int ins = 2 * wave + u;
+            const int L = 4 * ins + lline, P = lpos ^ (L & 15);
+            const int j = 2 * L + (P >> 3), qc = P & 7;
+            const void* src;
+            if constexpr (BF)
+                src = reinterpret_cast<const __bf16*>(Kt) + (qc >= 4 ? kt_lo_off : 0) +
+                      (int64_t)(r0 + j) * Mk + k0 + 8 * (qc & 3);
+            else
+                src = reinterpret_cast<const float*>(Kt) + (int64_t)(r0 + j) * Mk + k0 + 4 * qc;
+            lds_dma16(src, slot + LPR_TBYTES + ins * 1024);
+        }
+    };
+
+    f4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+
+    for (int q = 0; q < LPR_NS - 1 && q < nst; ++q) issue(q);
+    for (int q = 0; q < nst; ++q) {
+        // this wave's loads of stage q are done once at most the 6 of stage q+1 pend
+        if (q + 1 < nst) TKR_WAIT(6);
+        else TKR_WAIT(0);
+        __asm__ volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        __asm__ volatile("" ::: "memory");
+        if (q + LPR_NS - 1 < nst) issue(q + LPR_NS - 1);
+        const unsigned char* slot = ring + (q % LPR_NS) * LPR_SLOT;
+        const unsigned char* kb = slot + LPR_TBYTES;
+        float tv[4][8];
+        if constexpr (LEFT) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const f4 x = *reinterpret_cast<const f4*>(slot + (8 * g + e) * 1024 + (64 * wq + 4 * c) * 4);
+                tv[0][e] = x.x; tv[1][e] = x.y; tv[2][e] = x.z; tv[3][e] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = 64 * wq + 16 * b + c;
+                const f4 x0 = *reinterpret_cast<const f4*>(slot + lpr_chunk(j, 2 * g));
+                const f4 x1 = *reinterpret_cast<const f4*>(slot + lpr_chunk(j, 2 * g + 1));
+                tv[b][0] = x0.x; tv[b][1] = x0.y; tv[b][2] = x0.z; tv[b][3] = x0.w;
+                tv[b][4] = x1.x; tv[b][5] = x1.y; tv[b][6] = x1.z; tv[b][7] = x1.w;
+            }
+        }
+        if constexpr (BF) {
+            bf8 khi[4], klo[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int j = 64 * wr + 16 * a + c;
+                khi[a] = *reinterpret_cast<const bf8*>(kb + lpr_chunk(j, g));
+                klo[a] = *reinterpret_cast<const bf8*>(kb + lpr_chunk(j, 4 + g));
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                bf8 bh, bl;
+                split8(tv[b], bh, bl);
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(klo[a], bh, acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(khi[a], bl, acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(khi[a], bh, acc[a][b], 0, 0, 0);
+                }
+            }
+        } else {
+            float kv[4][8];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int j = 64 * wr + 16 * a + c;
+                const f4 x0 = *reinterpret_cast<const f4*>(kb + lpr_chunk(j, 2 * g));
+                const f4 x1 = *reinterpret_cast<const f4*>(kb + lpr_chunk(j, 2 * g + 1));
+                kv[a][0] = x0.x; kv[a][1] = x0.y; kv[a][2] = x0.z; kv[a][3] = x0.w;
+                kv[a][4] = x1.x; kv[a][5] = x1.y; kv[a][6] = x1.z; kv[a][7] = x1.w;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(kv[a][e], tv[b][e], acc[a][b], 0, 0, 0);
+        }
+    }
+
+    double* o = out + (int64_t)blockIdx.z * split_stride;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + 64 * wr + 16 * a + 4 * g + v;
+            if (r >= R) continue;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int64_t row = LEFT ? n0 + 64 * wq + 4 * c + b : n0 + 64 * wq + 16 * b + c;
+                o[row + out_ld * r] = (double)acc[a][b][v];
+            }
+        }
+}
+
 // Converted copy of a column-major factor (I x R, n = I R elements) for the
 // reduced-precision GEMMs: FP32, or BF16 hi (n elements) followed by BF16 lo.
 __global__ __launch_bounds__(256) void k_convert_factor(const double* __restrict__ A, int64_t n,
@@ -1519,8 +1726,71 @@ int launch_right_gemm_impl(itcpd_plan* p, MSrc* ms) {
 }
 
 
+// The LDS ring form of a reduced-precision GEMM applies: FP32-stored T, 256-row
+// (left) / 256-column (right) tiles and 32-index stages of the reduction.
+static bool lp_ring_ok(const itcpd_plan* p, bool left) {
+    const int64_t M = left ? p->ML : p->MR, K = left ? p->MR : p->ML;
+    return p->elt_bytes == 4 && p->Kt && M % 256 == 0 && K % 32 == 0;
+}
+
+static int launch_lp_ring(itcpd_plan* p, MSrc* ms, bool left) {
+    const bool bf = p->prec == ITCPD_BF16X3;
+    const Side sd = left ? make_side(p, p->s, p->N - p->s, /*rank_fast=*/false)
+                         : make_side(p, 0, p->s, /*rank_fast=*/false);
+    const int64_t M = left ? p->ML : p->MR;    // output rows
+    const int64_t Mk = left ? p->MR : p->ML;   // reduction length
+    const int ny = (int)cdiv(p->R, 128), Rk = 128 * ny;
+    const int64_t nk = Mk * Rk;
+    if (bf)
+        hipLaunchKernelGGL(k_krp_lp<true>, dim3((unsigned)cdiv(nk, 256)), dim3(256), 0, p->stream, sd, Mk,
+                           (int)p->R, Rk, (void*)p->Kt, p->chk);
+    else
+        hipLaunchKernelGGL(k_krp_lp<false>, dim3((unsigned)cdiv(nk, 256)), dim3(256), 0, p->stream, sd, Mk,
+                           (int)p->R, Rk, (void*)p->Kt, p->chk);
+    ITCPD_HIP(hipGetLastError());
+
+    // split-K factor: one workgroup per CU, so the fewest rounds per unit of work,
+    // plus a small charge per partial for summing them
+    const int64_t nstages = Mk / 32;
+    const int64_t base = (M / 256) * ny;
+    const int64_t out_elems = M * p->R;
+    int64_t nz = 1;
+    double best = 1e30;
+    for (int64_t z = 1; z <= 8; ++z) {
+        if (z > 1 && ((size_t)(z * out_elems) > p->parts_elems || nstages / z < 8)) break;
+        const double cost = (double)cdiv(base * z, p->num_cu) / (double)z + (z > 1 ? 0.005 * (double)z : 0.0);
+        if (cost < best - 1e-9) { best = cost; nz = z; }
+    }
+    const int64_t kper = cdiv(nstages, nz);
+    nz = cdiv(nstages, kper);
+    double* dst = left ? (p->s > 1 ? p->PL : p->M) : (p->N - p->s > 1 ? p->PR : p->M);
+    double* out = nz > 1 ? p->parts : dst;
+    {
+        Timed tm(p, left ? 0 : 1, 2.0 * (double)p->ML * (double)p->MR * (double)p->R,
+                 (double)p->ML * (double)p->MR * p->elt_bytes);
+        const dim3 grid((unsigned)(M / 256), (unsigned)ny, (unsigned)nz);
+        const float* T = (const float*)p->T;
+        const int64_t lo = Mk * Rk;
+        if (left && bf)
+            hipLaunchKernelGGL((k_lp_ring<true, true>), grid, dim3(512), LPR_LDS, p->stream, T, p->ML,
+                               (const void*)p->Kt, Mk, lo, kper, nstages, (int)p->R, out, M, out_elems, p->chk);
+        else if (left)
+            hipLaunchKernelGGL((k_lp_ring<true, false>), grid, dim3(512), LPR_LDS, p->stream, T, p->ML,
+                               (const void*)p->Kt, Mk, lo, kper, nstages, (int)p->R, out, M, out_elems, p->chk);
+        else if (bf)
+            hipLaunchKernelGGL((k_lp_ring<false, true>), grid, dim3(512), LPR_LDS, p->stream, T, p->ML,
+                               (const void*)p->Kt, Mk, lo, kper, nstages, (int)p->R, out, M, out_elems, p->chk);
+        else
+            hipLaunchKernelGGL((k_lp_ring<false, false>), grid, dim3(512), LPR_LDS, p->stream, T, p->ML,
+                               (const void*)p->Kt, Mk, lo, kper, nstages, (int)p->R, out, M, out_elems, p->chk);
+        ITCPD_HIP(hipGetLastError());
+    }
+    return finish_parts(p, p->parts, nz, out_elems, dst, ms, left ? p->s == 1 : p->N - p->s == 1);
+}
+
 // Reduced-precision forms of the two GEMMs (FP32 or BF16x3 arithmetic, FP64 output).
 int launch_left_gemm_lp(itcpd_plan* p, MSrc* ms) {
+    if (lp_ring_ok(p, true)) return launch_lp_ring(p, ms, true);
     const Side sd = make_side(p, p->s, p->N - p->s, /*rank_fast=*/false);
     const int RB = rb_lp(p), MB = mb_lp(RB, p->prec == ITCPD_BF16X3, p->elt_bytes);
     int rc;
@@ -1553,6 +1823,7 @@ int launch_left_gemm_lp(itcpd_plan* p, MSrc* ms) {
 }
 
 int launch_right_gemm_lp(itcpd_plan* p, MSrc* ms) {
+    if (lp_ring_ok(p, false)) return launch_lp_ring(p, ms, false);
     const Side sd = make_side(p, 0, p->s, /*rank_fast=*/false);
     const int RB = rb_lp(p), MB = mb_lp(RB, p->prec == ITCPD_BF16X3, p->elt_bytes);
     int rc;

@@ -42,7 +42,11 @@ def collinear(dims, R, c, noise, seed, dtype=np.float32):
 HOOK_CASES = [((64, 64, 64), 32, np.float32), ((32, 32, 32, 32), 64, np.float32),
               ((20, 17, 13, 9), 37, np.float32), ((64, 48, 40), 20, np.float64),
               ((96, 64, 32), 128, np.float32), ((128, 128, 128), 8, np.float32),
-              ((33, 40, 7), 3, np.float32)]
+              ((33, 40, 7), 3, np.float32),
+              # FP32-stored with 256-row / 256-column tiles: the LDS ring GEMMs (rank
+              # blocks of 128, padded; two blocks for R = 160)
+              ((16, 16, 16, 16), 160, np.float32), ((256, 24, 40), 45, np.float32),
+              ((32, 8, 32, 32), 128, np.float32)]
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
