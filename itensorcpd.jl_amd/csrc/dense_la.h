@@ -78,6 +78,20 @@ __device__ __forceinline__ int wave_first_max(double v, double& vmax) {
     return hit ? (int)__builtin_ctzll(hit) : 64;
 }
 
+// The same over two candidates per lane, a (index = lane) and b (index = 64 + lane):
+// the first index of the wave maximum among 128, 128 if none.
+__device__ __forceinline__ int wave_first_max2(double a, double b, double& vmax) {
+    double r = max_dpp_step<0x128>(fmax(a, b));
+    r = max_dpp_step<0x124>(r);
+    r = max_dpp_step<0x122>(r);
+    r = max_dpp_step<0x121>(r);
+    vmax = fmax(fmax(lane_value(r, 0), lane_value(r, 16)), fmax(lane_value(r, 32), lane_value(r, 48)));
+    const uint64_t ha = __builtin_amdgcn_ballot_w64(a == vmax);
+    if (ha) return (int)__builtin_ctzll(ha);
+    const uint64_t hb = __builtin_amdgcn_ballot_w64(b == vmax);
+    return hb ? 64 + (int)__builtin_ctzll(hb) : 128;
+}
+
 // ----------------------------------------------------- batched global loads
 // Small kernels here are latency bound: every helper issues all of its loads
 // before the first use, so a thread waits on memory once per batch.

@@ -874,6 +874,7 @@ constexpr int LPR_TBYTES = 32 * 1024;
 constexpr int LPR_KBYTES = 128 * 128;
 constexpr int LPR_SLOT = LPR_TBYTES + LPR_KBYTES;
 constexpr int LPR_LDS = LPR_NS * LPR_SLOT;
+constexpr int LPR_FLUSH = 4;
 
 // byte offset of chunk q (16 bytes) of 128-byte row j in a paired-line region
 __device__ __forceinline__ int lpr_chunk(int j, int q) {
@@ -961,11 +962,18 @@ __global__ __launch_bounds__(512, 1) void k_lp_ring(const float* __restrict__ T,
         }
     };
 
+    // FP32: the FP32 accumulators are added into FP64 ones every LPR_FLUSH stages, so
+    // no FP32 sum runs over more than 128 reduction indices (32 MFMA accumulations)
     f4 acc[4][4];
+    double tot[4][4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+        for (int b = 0; b < 4; ++b) {
+            acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) tot[a][b][v] = 0.0;
+        }
 
     for (int q = 0; q < LPR_NS - 1 && q < nst; ++q) issue(q);
     for (int q = 0; q < nst; ++q) {
@@ -1031,6 +1039,16 @@ __global__ __launch_bounds__(512, 1) void k_lp_ring(const float* __restrict__ T,
 #pragma unroll
                     for (int b = 0; b < 4; ++b)
                         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(kv[a][e], tv[b][e], acc[a][b], 0, 0, 0);
+            if ((q & (LPR_FLUSH - 1)) == LPR_FLUSH - 1 || q + 1 == nst) {
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) tot[a][b][v] += (double)acc[a][b][v];
+                        acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+                    }
+            }
         }
     }
 
@@ -1044,7 +1062,7 @@ __global__ __launch_bounds__(512, 1) void k_lp_ring(const float* __restrict__ T,
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int64_t row = LEFT ? n0 + 64 * wq + 4 * c + b : n0 + 64 * wq + 16 * b + c;
-                o[row + out_ld * r] = (double)acc[a][b][v];
+                o[row + out_ld * r] = BF ? (double)acc[a][b][v] : tot[a][b][v];
             }
         }
 }

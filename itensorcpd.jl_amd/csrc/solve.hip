@@ -111,6 +111,132 @@ __global__ __launch_bounds__(FACTOR_THREADS) void k_factor(GramSet gs, int mode,
     qr_fallback<NT>(gs, mode, R, W, qtau, ztau, jpvt, vwork, finfo, redv, redi, rank);
 }
 
+// Pivoted Cholesky for 64 < R <= 128: the register form of factor64_dev on 16
+// waves.  Thread (w, l) holds rows 16 (w >> 1) .. + 15 of column l + 64 (w & 1)
+// of the Schur complement (V from the Grams, straight into registers); each step
+// the two waves owning row p hand it to all waves through LDS and every thread
+// applies the rank-1 update (one step late, as in factor64_dev).  Every wave runs
+// the pivot search over all 128 candidates (columns l and l + 64 of each lane) on
+// identical data.  Same pivot rule and outputs as k_factor: Uperm(j, t) =
+// U(j, piv t) in pivoted order, invdiag, piv; on rank deficiency the QR data.
+constexpr int F128_LDT = 129;
+constexpr size_t F128_LDS = (size_t)128 * F128_LDT * 8;
+
+__global__ __launch_bounds__(1024) void k_factor128(GramSet gs, int mode, int R, double tol,
+                                                    double* __restrict__ Uperm,
+                                                    double* __restrict__ invdiag,
+                                                    int* __restrict__ piv, int* __restrict__ finfo,
+                                                    double* __restrict__ W, double* __restrict__ qtau,
+                                                    double* __restrict__ ztau, int* __restrict__ jpvt,
+                                                    double* vwork, const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    constexpr int LDT = F128_LDT;
+    extern __shared__ double UbT[];   // UbT(l, j) = U(j, l), 128 x LDT
+    __shared__ double diagU[128];
+    __shared__ double invdU[128];
+    __shared__ double rowb[2][128];
+    __shared__ int pivS[128];
+    __shared__ int s_rank;
+    __shared__ double redv[1024];
+    __shared__ int redi[1024];
+    const int t = threadIdx.x, l = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int h = w & 1, row0 = 16 * (w >> 1);
+    const int col = l + 64 * h;
+
+    d16 sr;
+    double vda = 0.0, vdb = 0.0;
+    {
+        double v[16];
+        double da = 1.0, db = 1.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = 1.0;
+        for (int m = 0; m < gs.N; ++m) {
+            if (m == mode) continue;
+            const double* __restrict__ G = gs.G[m];
+            double gv[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int row = row0 + i;
+                gv[i] = (row < R && col < R) ? G[row * R + col] : 0.0;
+            }
+            const double ga = l < R ? G[l * R + l] : 0.0;
+            const double gb = l + 64 < R ? G[(l + 64) * R + l + 64] : 0.0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v[i] *= gv[i];
+            da *= ga;
+            db *= gb;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sr[i] = v[i];
+        vda = l < R ? da : 0.0;
+        vdb = l + 64 < R ? db : 0.0;
+    }
+    // remaining indices: bit l of maska (index l) and of maskb (index 64 + l)
+    uint64_t maska = R >= 64 ? ~0ull : ((1ull << R) - 1);
+    uint64_t maskb = R >= 128 ? ~0ull : ((1ull << (R - 64)) - 1);
+    double suma = 0.0, sumb = 0.0, cu = 0.0;
+    int rank = R;
+    // rc: the pending row (step j-1's row p, this thread's 16 rows), applied with cu
+    d16 rc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) rc[i] = 0.0;
+    for (int j = 0; j < R; ++j) {
+        double d;
+        const bool rema = (maska >> l) & 1, remb = (maskb >> l) & 1;
+        const int p = wave_first_max2(rema ? vda - suma : -__builtin_inf(),
+                                      remb ? vdb - sumb : -__builtin_inf(), d);
+        const bool live = p < 64 ? ((maska >> p) & 1) : (p < 128 && ((maskb >> (p - 64)) & 1));
+        const bool bad = (p >= R) || !live || ((j == 0) ? !(d > 0.0) : !(d > tol));
+        if (bad) { rank = j; break; }
+        if ((w >> 1) == (p >> 4)) rowb[j & 1][col] = fma(-rc[p & 15], cu, sr[p & 15]);
+        if (p < 64) maska &= ~(1ull << p);
+        else maskb &= ~(1ull << (p - 64));
+        __syncthreads();
+        const double* rb = rowb[j & 1];
+        const double rla = rb[l], rlb = rb[l + 64];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sr[i] = fma(-rc[i], cu, sr[i]);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) rc[i] = rb[row0 + i];
+        // 1/sqrt(d): hardware estimate and two Newton steps (as factor64_dev)
+        const double dc = fmax(d, 0x1p-1020);
+        const double hh = 0.5 * dc;
+        double y = __builtin_amdgcn_rsq(dc);
+        y = fma(y, fma(-hh * y, y, 0.5), y);
+        y = fma(y, fma(-hh * y, y, 0.5), y);
+        const double inv = y, ujj = dc * y;
+        const double ua = ((maska >> l) & 1) ? rla * inv : 0.0;
+        const double ub = ((maskb >> l) & 1) ? rlb * inv : 0.0;
+        suma = fma(ua, ua, suma);
+        sumb = fma(ub, ub, sumb);
+        cu = (h ? ub : ua) * inv;
+        if (t == 0) { diagU[j] = ujj; invdU[j] = inv; pivS[j] = p; }
+        if (w == 0) {
+            if (l < R) UbT[l * LDT + j] = ua;
+            if (l + 64 < R) UbT[(l + 64) * LDT + j] = ub;
+        }
+    }
+    if (t == 0) s_rank = rank;
+    __syncthreads();
+    if (s_rank == R) {
+        for (int j = t; j < R; j += 1024) {
+            invdiag[j] = invdU[j];
+            piv[j] = pivS[j];
+        }
+        for (int x = t; x < R * R; x += 1024) {
+            const int j = x / R, tt = x - j * R;
+            double v = 0.0;
+            if (tt > j) v = UbT[pivS[tt] * LDT + j];
+            else if (tt == j) v = diagU[j];
+            Uperm[x] = v;
+        }
+        if (t == 0) { finfo[0] = R; finfo[1] = 0; finfo[2] = R; }
+        return;
+    }
+    qr_fallback<1024>(gs, mode, R, W, qtau, ztau, jpvt, vwork, finfo, redv, redi, s_rank);
+}
+
 __global__ __launch_bounds__(256) void k_factor64(GramSet gs, int mode, int R, double tol,
                                                   double* __restrict__ Uperm,
                                                   double* __restrict__ Vinv,
@@ -925,6 +1051,9 @@ int launch_factor_on(itcpd_plan* p, int mode, hipStream_t st) {
         hipLaunchKernelGGL(k_factor64, dim3(1), dim3(256), FACTOR64_LDS,
                            st, grams_of(p), mode, R, 1e-6, p->U, p->Upad, p->invdiag, p->piv,
                            p->finfo, W, qtau, ztau, p->piv + R, vwork, p->chk);
+    } else if (R <= 128) {
+        hipLaunchKernelGGL(k_factor128, dim3(1), dim3(1024), F128_LDS, st, grams_of(p), mode, R, 1e-6,
+                           p->U, p->invdiag, p->piv, p->finfo, W, qtau, ztau, p->piv + R, vwork, p->chk);
     } else {
         // right-looking, all threads; V in LDS up to R = 128, else in the U buffer
         const bool lds = R <= 128;
