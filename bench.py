@@ -186,6 +186,23 @@ def bench_precision(args, E) -> dict:
             plan.set_cpd(init)
             runs[name] = run_schedule(plan, E, nrm, sched, args.max_sweeps, 1e-9)
             print(f"[bench] {name}: {len(runs[name])} sweeps", file=sys.stderr, flush=True)
+        # the ring GEMMs of the reduced forms, timed with HIP events on the plan stream
+        kstats = {}
+        for prec in ("fp32", "bf16x3"):
+            plan.set_precision(prec)
+            plan.set_cpd(init)
+            plan.run(E.NoCheck(1))
+            plan.synchronize()
+            plan.set_profiling(True)
+            plan.run(E.NoCheck(3))
+            plan.synchronize()
+            for side, kid in (("left", 0), ("right", 1)):
+                st = plan.kernel_stats(kid)
+                avg = st["total_ms"] / st["launches"]
+                kstats[f"{prec}_{side}"] = {"launches": st["launches"], "avg_ms": avg,
+                                            "tflops": st["flops"] / st["launches"] / (avg * 1e-3) / 1e12,
+                                            "T_GBps": 4.0 * I ** 4 / (avg * 1e-3) / 1e9}
+            plan.set_profiling(False)
         plan.set_precision("fp64")
         fit_star = runs["plain FP64"][-1][1]
         res = {}
@@ -217,8 +234,13 @@ def bench_precision(args, E) -> dict:
         "config": {"workload": "precision forms: 224^4 FP32-stored, rank 128, random_CPD init, "
                                "sweeps until the FP64 fit is within 1e-6 of the converged FP64 fit",
                    "fit_star": fit_star, "forms": res},
-        "roofline": {"bound": "hbm", "achieved": None, "peak": 8000.0, "unit": "GB/s", "frac": None,
-                     "traffic": None,
+        # dominant kernel of the mixed run: the FP32 ring GEMM (MFMA-bound; most of the
+        # mixed time-to-fit is FP32 sweeps); the BF16x3 one is HBM-bound on T
+        "roofline": {"bound": "mfma", "achieved": kstats["fp32_left"]["tflops"], "peak": 157.3,
+                     "unit": "TFLOP/s", "frac": kstats["fp32_left"]["tflops"] / 157.3,
+                     "traffic": None, "kernel": "FP32 left ring GEMM (2*I^4*R flop/launch)",
+                     "kernels": kstats,
+                     "bf16x3_left_hbm_frac": kstats["bf16x3_left"]["T_GBps"] / 8000.0,
                      "sweep_fp64_tflops": flops / (ms_plain * 1e-3) / 1e12,
                      "sweep_bf16x3_tflops_fp32_equiv": flops / (mixed["ms_per_sweep"].get("bf16x3", float("nan")) * 1e-3) / 1e12,
                      "sweep_bf16x3_tensor_GBps": 2 * 4 * I ** 4 / (mixed["ms_per_sweep"].get("bf16x3", float("nan")) * 1e-3) / 1e9},
