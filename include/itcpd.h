@@ -163,6 +163,35 @@ int itcpd_fit(itcpd_plan* plan, const double* M_last, int32_t mem, double ref_no
 /* norm(T) on the device (the ref_norm handed to FitCheck). */
 int itcpd_tensor_norm(itcpd_plan* plan, double* norm_out);
 
+/* ---- leverage-score-sampled ALS (LevScoreSampled) ----
+ * compute_leverage_score_probabilitiy (math_tools/probability.jl:3-10) of the plan's
+ * factor `mode` (I_n x R): p(i) = sum_j Q(i, j)^2 / min(I_n, R), Q the thin Q of its
+ * unpivoted Householder QR.  probs: I_n doubles.  These are the weights the
+ * sampler draws each mode's indices from (probability.jl:12-33). */
+int itcpd_leverage_scores(itcpd_plan* plan, int32_t mode, double* probs, int32_t mem);
+
+/* One projected least-squares step of a ProjectionAlgorithm for `mode`, as the
+ * optimize loop runs it (optimizers/als_optimizers/optimize.jl:17-25 with
+ * ProjectionAlgorithm.jl:4-8,54-67 and krp_lev_score_sampled.jl:18-49):
+ *   K_s = pivot_hadamard(the other factors, samples)     (algebra/had_contract.jl:277-295)
+ *   T_s = fused_flatten_sample(T, mode, samples)         (algebra/pivot_mapping.jl:90-118)
+ *   normal != 0: X^T = ldiv_solve(K_s^T K_s, (T_s K_s)^T) (pivoted Cholesky, tol 1e-6,
+ *                with the column-pivoted QR fallback)
+ *   normal == 0: X^T = qr(K_s, ColumnNorm()) \ T_s^T     (minimum-norm least squares)
+ * then row_norm into the plan's CPD (A_n, lambda, G_n).
+ * samples: nsamps x (N-1) column-major, 0-based indices of the other modes in mode
+ * order (the reference's 1-based `sampled_cols` minus one); every index is checked.
+ * info (nullable, 2 ints): [0] = 1 when the normal form's Cholesky fell back to QR,
+ * [1] = numerical rank of the solve. */
+int itcpd_sampled_solve(itcpd_plan* plan, int32_t mode, int64_t nsamps, const int32_t* samples,
+                        int32_t mem, int32_t normal, int32_t* info);
+
+/* check_converge of a ProjectionAlgorithm with a CPDiffCheck or CPAngleCheck
+ * (ProjectionAlgorithm.jl:53 -> cp_diff_check.jl:20-71 / cp_angle_check.jl:20-73):
+ * one evaluation of the check on the plan's current CPD, its previous CPD held by
+ * the plan as in itcpd_run.  check->converged = the check's `true` (stop). */
+int itcpd_check_cpd(itcpd_plan* plan, itcpd_check* check);
+
 /* Switches the MTTKRP arithmetic of an existing plan (no counterpart in the
  * reference): the CPD, Grams and check state are kept, so a run can continue
  * in another precision (e.g. BF16x3 sweeps, then FP32 or FP64 ones). */

@@ -10,12 +10,14 @@ a test written like the reference's ``test/cp_als.jl``) reads the same:
     FitCheck(tol, max, norm), NoCheck(max), CPDiffCheck(tol, max),
     CPAngleCheck(tol, max), CPDFit(check)                      converge_checks/*.jl
     KRPFreeNormal(), KRPNormal()                               algorithms/.../standard/tensor.jl
+    LevScoreSampled(n); als_optimize(...; normal, stop_resample) randomized/krp_lev_score_sampled.jl
     CPD (factors, lam), cp_rank                                cpd.jl:7-46
 
 All arithmetic runs in libitcpd_hip.so on the GPU; there is no CPU fallback.
 """
 from .api import (  # noqa: F401
     CPD, ALSPlan, BatchALS, DeviceTensor, MLP, reconstruct, CPDFit, FitCheck, CPDiffCheck, CPAngleCheck, KRPFreeNormal, KRPNormal, NoCheck, CPDOptimizer,
+    LevScoreSampled, ProjectionAlgorithm,
     als_optimize, cp_rank, decompose, decompose_adaptive, decompose_batch, increase_cpd_rank,
     random_CPD, tensor_norm,
 )
@@ -23,6 +25,7 @@ from ._lib import ITCPDError  # noqa: F401
 
 __all__ = [
     "CPD", "ALSPlan", "BatchALS", "decompose_batch", "DeviceTensor", "MLP", "reconstruct", "CPDFit", "FitCheck", "CPDiffCheck", "CPAngleCheck", "KRPFreeNormal", "KRPNormal", "NoCheck",
+    "LevScoreSampled", "ProjectionAlgorithm",
     "CPDOptimizer", "als_optimize", "cp_rank", "decompose", "decompose_adaptive",
     "increase_cpd_rank", "random_CPD", "tensor_norm", "ITCPDError",
 ]

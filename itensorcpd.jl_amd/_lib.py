@@ -59,6 +59,10 @@ SIGNATURES = {
     "itcpd_row_norm_post_solve": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
     "itcpd_fit": (C.c_int, [_P, _P, C.c_int32, C.c_double, _DP, _DP, _DP]),
     "itcpd_tensor_norm": (C.c_int, [_P, _DP]),
+    "itcpd_leverage_scores": (C.c_int, [_P, C.c_int32, _P, C.c_int32]),
+    "itcpd_sampled_solve": (C.c_int, [_P, C.c_int32, C.c_int64, _P, C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_int32)]),
+    "itcpd_check_cpd": (C.c_int, [_P, C.POINTER(CheckStruct)]),
     "itcpd_plan_set_precision": (C.c_int, [_P, C.c_int32]),
     "itcpd_plan_set_profiling": (C.c_int, [_P, C.c_int32]),
     "itcpd_get_kernel_stats": (C.c_int, [_P, C.c_int32, C.POINTER(KernelStats)]),

@@ -31,6 +31,22 @@ class KRPNormal(MttkrpAlgorithm):
     """algorithms/als_algorithms/standard/tensor.jl:10."""
 
 
+class ProjectionAlgorithm:
+    """ProjectionAlgorithm.jl:1-3: solvers of a sampled / projected least-squares problem
+    f = |P T - [[A, B, C]] P|^2 instead of the normal equations."""
+
+
+class LevScoreSampled(ProjectionAlgorithm):
+    """krp_lev_score_sampled.jl:9-16: leverage-score sampling of the Khatri-Rao rows,
+    NSamples = one sample count for every mode or one per mode."""
+
+    def __init__(self, nsamples=1):
+        self.NSamples = (int(nsamples),) if np.isscalar(nsamples) else tuple(int(n) for n in nsamples)
+
+    def nsamples(self, mode: int) -> int:
+        return self.NSamples[0] if len(self.NSamples) == 1 else self.NSamples[mode]
+
+
 def _alg_code(alg) -> int:
     if alg is None or isinstance(alg, KRPFreeNormal) or alg is KRPFreeNormal:
         return L.ITCPD_ALG_KRP_FREE_NORMAL
@@ -284,6 +300,50 @@ def _prec_code(prec) -> int:
     return int(prec)
 
 
+def _check_struct(check) -> "L.CheckStruct":
+    """The C-ABI view of a convergence check's state."""
+    cs = L.CheckStruct()
+    if isinstance(check, FitCheck):
+        cs.kind = L.ITCPD_CHECK_FIT
+        cs.counter = check.counter
+        cs.tolerance = check.tolerance
+        cs.ref_norm = check.ref_norm
+        cs.total_iter = check.total_iter
+        cs.final_fit = check.final_fit
+    elif isinstance(check, NoCheck):
+        cs.kind = L.ITCPD_CHECK_NONE
+    elif isinstance(check, (CPDiffCheck, CPAngleCheck)):
+        cs.kind = L.ITCPD_CHECK_CPDIFF if isinstance(check, CPDiffCheck) else L.ITCPD_CHECK_CPANGLE
+        cs.counter = check.counter
+        cs.tolerance = check.tolerance
+        cs.ref_norm = check.norm_prev_iter
+        cs.has_prev = 1 if check.has_prev else 0
+        cs.total_iter = check.total_iter
+        cs.final_fit = check.final_fit
+    else:
+        raise NotImplementedError(f"check {type(check).__name__} is not on the GPU path")
+    cs.iter = check.iter
+    cs.max_counter = check.max_counter
+    cs.lastfit = check.lastangle if isinstance(check, CPAngleCheck) else check.lastfit
+    return cs
+
+
+def _check_update(check, cs) -> None:
+    """Copies the C-ABI check state back into the Python check."""
+    check.iter = cs.iter
+    if isinstance(check, CPAngleCheck):
+        check.lastangle = cs.lastfit
+    else:
+        check.lastfit = cs.lastfit
+    if not isinstance(check, NoCheck):
+        check.counter = cs.counter
+        check.total_iter = cs.total_iter
+        check.final_fit = cs.final_fit
+    if isinstance(check, (CPDiffCheck, CPAngleCheck)):
+        check.norm_prev_iter = cs.ref_norm
+        check.has_prev = bool(cs.has_prev)
+
+
 class ALSPlan:
     """A device-resident ALS problem (the ``ALS`` struct, als_optimizer.jl:5-10).
 
@@ -358,30 +418,7 @@ class ALSPlan:
 
     # -- the sweep loop (optimize.jl:6-35)
     def run(self, check, verbose: bool = False) -> dict:
-        cs = L.CheckStruct()
-        if isinstance(check, FitCheck):
-            cs.kind = L.ITCPD_CHECK_FIT
-            cs.counter = check.counter
-            cs.tolerance = check.tolerance
-            cs.ref_norm = check.ref_norm
-            cs.total_iter = check.total_iter
-            cs.final_fit = check.final_fit
-        elif isinstance(check, NoCheck):
-            cs.kind = L.ITCPD_CHECK_NONE
-        elif isinstance(check, (CPDiffCheck, CPAngleCheck)):
-            cs.kind = L.ITCPD_CHECK_CPDIFF if isinstance(check, CPDiffCheck) else L.ITCPD_CHECK_CPANGLE
-            cs.counter = check.counter
-            cs.tolerance = check.tolerance
-            cs.ref_norm = check.norm_prev_iter
-            cs.has_prev = 1 if check.has_prev else 0
-            cs.total_iter = check.total_iter
-            cs.final_fit = check.final_fit
-        else:
-            raise NotImplementedError(f"check {type(check).__name__} is not on the GPU path")
-        angle = isinstance(check, CPAngleCheck)
-        cs.iter = check.iter
-        cs.max_counter = check.max_counter
-        cs.lastfit = check.lastangle if angle else check.lastfit
+        cs = _check_struct(check)
         iter0, lastfit0 = check.iter, cs.lastfit
         nmax = max(check.max_counter - check.iter, 0)
         trace = np.zeros(max(nmax, 1))
@@ -389,18 +426,7 @@ class ALSPlan:
                                  trace.ctypes.data_as(C.POINTER(C.c_double)))
         if rc not in (L.ITCPD_OK, L.ITCPD_E_NAN):
             L.check(rc)
-        check.iter = cs.iter
-        if angle:
-            check.lastangle = cs.lastfit
-        else:
-            check.lastfit = cs.lastfit
-        if not isinstance(check, NoCheck):
-            check.counter = cs.counter
-            check.total_iter = cs.total_iter
-            check.final_fit = cs.final_fit
-        if isinstance(check, (CPDiffCheck, CPAngleCheck)):
-            check.norm_prev_iter = cs.ref_norm
-            check.has_prev = bool(cs.has_prev)
+        _check_update(check, cs)
         fits = trace[:cs.sweeps_run].copy() if not isinstance(check, NoCheck) else np.zeros(0)
         if verbose:
             _print_trace(check, self.rank, iter0, lastfit0, fits, cs.sweeps_run)
@@ -440,6 +466,38 @@ class ALSPlan:
         L.check(self._lib.itcpd_fit(self._h, C.c_void_p(M_last.ctypes.data), L.ITCPD_MEM_HOST,
                                     float(ref_norm), C.byref(inner), C.byref(fs), C.byref(fit)))
         return inner.value, fs.value, fit.value
+
+    # -- leverage-score-sampled steps (LevScoreSampled)
+    def leverage_scores(self, mode: int) -> np.ndarray:
+        """compute_leverage_score_probabilitiy (probability.jl:3-10) of factor `mode`."""
+        out = np.empty(self.dims[mode])
+        L.check(self._lib.itcpd_leverage_scores(self._h, mode, C.c_void_p(out.ctypes.data),
+                                                L.ITCPD_MEM_HOST))
+        return out
+
+    def sampled_solve(self, mode: int, samples: np.ndarray, normal: bool = False):
+        """One projected least-squares step for `mode` with the 0-based sampled index
+        tuples of the other modes (nsamps x (N-1)); the result goes into the plan's CPD.
+        Returns (the normal form's Cholesky fell back to QR, rank of the solve)."""
+        smp = np.asfortranarray(samples, dtype=np.int32)
+        if smp.ndim != 2 or smp.shape[1] != len(self.dims) - 1:
+            raise ValueError("samples must be nsamps x (order - 1)")
+        info = (C.c_int32 * 2)()
+        L.check(self._lib.itcpd_sampled_solve(self._h, mode, smp.shape[0], C.c_void_p(smp.ctypes.data),
+                                              L.ITCPD_MEM_HOST, 1 if normal else 0, info))
+        return bool(info[0]), int(info[1])
+
+    def check_cpd(self, check) -> bool:
+        """One evaluation of a CPDiffCheck / CPAngleCheck on the current CPD (the check's
+        check_converge); True = stop."""
+        cs = _check_struct(check)
+        rc = self._lib.itcpd_check_cpd(self._h, C.byref(cs))
+        if rc not in (L.ITCPD_OK, L.ITCPD_E_NAN):
+            L.check(rc)
+        _check_update(check, cs)
+        if rc == L.ITCPD_E_NAN:
+            raise ALSNaNError("Error NAN")
+        return bool(cs.converged)
 
     def tensor_norm(self) -> float:
         out = C.c_double()
@@ -506,10 +564,69 @@ def random_CPD(target, rank: int, rng=None) -> CPD:
     return CPD(fs, lam)
 
 
-def als_optimize(target, cp: CPD, alg=None, check=None, maxiter=None, verbose=False) -> CPD:
+def _sample_factor_matrices(nsamps: int, skip: int, probs, rng) -> np.ndarray:
+    """probability.jl:12-33: nsamps independent draws (with replacement, weights
+    |p|) from each factor's probabilities except `skip`'s, in mode order; 0-based."""
+    cols = []
+    for m, p in enumerate(probs):
+        if m == skip:
+            continue
+        w = np.abs(p)
+        cols.append(rng.choice(len(w), size=nsamps, replace=True, p=w / w.sum()).astype(np.int32))
+    return np.stack(cols, axis=1)
+
+
+def _optimize_sampled(target, cp: CPD, alg: LevScoreSampled, check, normal: bool, stop_resample: int,
+                      rng, verbose: bool) -> CPD:
+    """compute_als(::LevScoreSampled) (optimizers/als_optimizers/randomized/
+    krp_lev_score_sampled.jl:1-37) and optimize (optimize.jl:6-32) with project_krp /
+    post_solve (algorithms/.../krp_lev_score_sampled.jl:18-57) and the
+    ProjectionAlgorithm check (ProjectionAlgorithm.jl:20-53).  The sampled KRP,
+    sampled fibers, solves, leverage scores and CP checks run on the device; the
+    draws are made here, as the reference makes them on the host (StatsBase)."""
+    rng = np.random.default_rng(rng)
+    N = len(cp.factors)
+    with ALSPlan(target, cp.rank) as plan:
+        plan.set_cpd(cp)
+        weights = [plan.leverage_scores(m) for m in range(N)]
+        samples = [_sample_factor_matrices(alg.nsamples(m), m, weights, rng) for m in range(N)]
+        it = check.iter
+        while it < check.max_counter:
+            for n in range(N):
+                if stop_resample < 0 or stop_resample > check.iter:
+                    samples[n] = _sample_factor_matrices(alg.nsamples(n), n, weights, rng)
+                plan.sampled_solve(n, samples[n], normal)
+                weights[n] = plan.leverage_scores(n)
+            if isinstance(check, FitCheck):
+                # a FitCheck only counts sweeps for these algorithms
+                check.iter += 1
+                if check.iter >= check.max_counter:
+                    check.iter = 0
+                stop = False
+            elif isinstance(check, NoCheck):
+                check.iter += 1
+                stop = check.iter == check.max_counter
+                if stop:
+                    check.iter = 0
+            else:
+                stop = plan.check_cpd(check)
+            if verbose:
+                print(f"{cp.rank}\t {check.iter}")
+            if stop:
+                break
+            it += 1
+        return plan.get_cpd()
+
+
+def als_optimize(target, cp: CPD, alg=None, check=None, maxiter=None, verbose=False, *,
+                 normal: bool = False, stop_resample: int = -1, rng=None) -> CPD:
     """als_optimizer.jl:15-25 with the defaults of compute_als (:37-55):
-    alg = KRPFreeNormal(), check = NoCheck(maxiter or 100).  Returns a new CPD."""
+    alg = KRPFreeNormal(), check = NoCheck(maxiter or 100).  Returns a new CPD.
+    With a ProjectionAlgorithm (LevScoreSampled) the sampled ALS runs instead; normal,
+    stop_resample are its compute_als keywords and rng seeds the draws."""
     check = NoCheck(100 if maxiter is None else maxiter) if check is None else check
+    if isinstance(alg, ProjectionAlgorithm):
+        return _optimize_sampled(target, cp, alg, check, normal, stop_resample, rng, verbose)
     with ALSPlan(target, cp.rank, alg=alg) as plan:
         plan.set_cpd(cp)
         plan.run(check, verbose=verbose)

@@ -340,6 +340,8 @@ int itcpd_plan_destroy(itcpd_plan* p) {
     dfree(p->fit_trace);
     for (int q = 0; q < ITCPD_MAX_ORDER; ++q) dfree(p->Atprev[q]);
     dfree(p->lamprev);
+    dfree(p->smp_ws);
+    dfree(p->smp_iws);
     dfree(p->xg_part);
     if (p->chk_host) (void)hipHostFree(p->chk_host);
     if (p->side_stream) (void)hipStreamSynchronize(p->side_stream);
@@ -434,6 +436,21 @@ int itcpd_get_cpd(itcpd_plan* p, double* const* factors, double* lambda, int32_t
     return ITCPD_OK;
 }
 
+// PrevCP storage of the CP checks and their cross-Gram partials (row chunks bounded
+// so that the partials stay within 2^24 doubles), allocated on first use.
+static int ensure_prevcp(itcpd_plan* p) {
+    if (p->lamprev) return ITCPD_OK;
+    int rc;
+    for (int q = 0; q < p->N; ++q)
+        if ((rc = dmalloc(&p->Atprev[q], (size_t)(p->dims[q] * p->Rp)))) return rc;
+    if ((rc = dmalloc(&p->lamprev, (size_t)p->Rp))) return rc;
+    const int64_t RR = p->R * p->R;
+    int64_t ch = ((int64_t)1 << 24) / (p->N * RR);
+    ch = std::max<int64_t>(1, std::min<int64_t>(ch, 16));
+    p->xg_chunks = (int)ch;
+    return dmalloc(&p->xg_part, (size_t)(p->N * ch * RR));
+}
+
 int itcpd_run(itcpd_plan* p, itcpd_check* c, double* fit_trace) {
     if (!p || !c) return fail(ITCPD_E_ARG, "null argument");
     if (!p->have_cpd) return fail(ITCPD_E_STATE, "itcpd_run before itcpd_set_cpd / itcpd_random_cpd");
@@ -444,18 +461,9 @@ int itcpd_run(itcpd_plan* p, itcpd_check* c, double* fit_trace) {
     if (cp_check && c->has_prev && !p->prev_valid)
         return fail(ITCPD_E_STATE, "check holds a previous CPD that this plan does not have");
     DeviceGuard dg(p->device);
-    if (cp_check && !p->lamprev) {
-        // PrevCP storage and cross-Gram partials (row chunks bounded so that the
-        // partials stay within 2^24 doubles)
-        int rc;
-        for (int q = 0; q < p->N; ++q)
-            if ((rc = dmalloc(&p->Atprev[q], (size_t)(p->dims[q] * p->Rp)))) return rc;
-        if ((rc = dmalloc(&p->lamprev, (size_t)p->Rp))) return rc;
-        const int64_t RR = p->R * p->R;
-        int64_t ch = ((int64_t)1 << 24) / (p->N * RR);
-        ch = std::max<int64_t>(1, std::min<int64_t>(ch, 16));
-        p->xg_chunks = (int)ch;
-        if ((rc = dmalloc(&p->xg_part, (size_t)(p->N * ch * RR)))) return rc;
+    if (cp_check) {
+        const int rc = ensure_prevcp(p);
+        if (rc) return rc;
     }
     // optimize.jl:8: the sweep loop runs while the local counter (starting at
     // check.iter) is below check.max_counter
@@ -625,6 +633,109 @@ int itcpd_fit(itcpd_plan* p, const double* M_last, int32_t mem, double ref_norm,
     if (inner) *inner = sc[0];
     if (fact_square) *fact_square = sc[1];
     if (fit) *fit = sc[2];
+    return ITCPD_OK;
+}
+
+int itcpd_leverage_scores(itcpd_plan* p, int32_t mode, double* probs, int32_t mem) {
+    if (!p || !probs) return fail(ITCPD_E_ARG, "null argument");
+    if (mode < 0 || mode >= p->N) return fail(ITCPD_E_ARG, "mode out of range");
+    if (!p->have_cpd) return fail(ITCPD_E_STATE, "no CPD set");
+    DeviceGuard dg(p->device);
+    int rc;
+    if ((rc = reset_check_flags(p))) return rc;
+    double* used = nullptr;
+    if ((rc = launch_leverage_scores(p, mode, mem == ITCPD_MEM_DEVICE ? probs : nullptr, &used))) return rc;
+    if (mem != ITCPD_MEM_DEVICE && (rc = copy_out(probs, used, (size_t)p->dims[mode] * 8, mem, p->stream)))
+        return rc;
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    return ITCPD_OK;
+}
+
+int itcpd_sampled_solve(itcpd_plan* p, int32_t mode, int64_t nsamps, const int32_t* samples, int32_t mem,
+                        int32_t normal, int32_t* info) {
+    if (!p || !samples) return fail(ITCPD_E_ARG, "null argument");
+    if (mode < 0 || mode >= p->N) return fail(ITCPD_E_ARG, "mode out of range");
+    if (p->N < 2) return fail(ITCPD_E_ARG, "sampling needs at least two modes");
+    if (nsamps < 1 || nsamps > ((int64_t)1 << 31) / p->N) return fail(ITCPD_E_ARG, "nsamps out of range");
+    if (!p->have_cpd) return fail(ITCPD_E_STATE, "no CPD set");
+    DeviceGuard dg(p->device);
+    int rc;
+    if ((rc = reset_check_flags(p))) return rc;
+    if ((rc = sampled_ws_prepare(p, mode, nsamps))) return rc;
+    // the indices address the factors and T: check every one on the host
+    const size_t nidx = (size_t)nsamps * (size_t)(p->N - 1);
+    std::vector<int32_t> h(nidx);
+    if (mem == ITCPD_MEM_DEVICE) {
+        ITCPD_HIP(hipMemcpyAsync(h.data(), samples, nidx * 4, hipMemcpyDeviceToHost, p->stream));
+        ITCPD_HIP(hipStreamSynchronize(p->stream));
+    } else {
+        std::memcpy(h.data(), samples, nidx * 4);
+    }
+    int q = 0;
+    for (int m = 0; m < p->N; ++m) {
+        if (m == mode) continue;
+        const int32_t* col = h.data() + (size_t)nsamps * q;
+        for (int64_t s = 0; s < nsamps; ++s)
+            if (col[s] < 0 || col[s] >= p->dims[m]) return fail(ITCPD_E_ARG, "sample index out of range");
+        ++q;
+    }
+    ITCPD_HIP(hipMemcpyAsync(p->smp_iws, h.data(), nidx * 4, hipMemcpyHostToDevice, p->stream));
+    if ((rc = launch_sampled_solve(p, mode, nsamps, p->smp_iws, normal != 0))) return rc;
+    if ((rc = launch_col_sumsq(p, mode, p->X))) return rc;
+    if ((rc = launch_normalize(p, mode, p->X, false))) return rc;
+    int fi[4] = {0, 0, 0, 0};
+    if (normal)
+        ITCPD_HIP(hipMemcpyAsync(fi, p->finfo, sizeof(fi), hipMemcpyDeviceToHost, p->stream));
+    else
+        ITCPD_HIP(hipMemcpyAsync(fi, p->smp_iws + nidx + p->R, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    if (info) {
+        info[0] = normal ? fi[1] : 0;            // the Cholesky fell back to the QR solve
+        info[1] = normal ? fi[2] : fi[0];        // rank of the solve
+    }
+    return ITCPD_OK;
+}
+
+int itcpd_check_cpd(itcpd_plan* p, itcpd_check* c) {
+    if (!p || !c) return fail(ITCPD_E_ARG, "null argument");
+    if (c->kind != ITCPD_CHECK_CPDIFF && c->kind != ITCPD_CHECK_CPANGLE)
+        return fail(ITCPD_E_ARG, "itcpd_check_cpd takes a CPDiffCheck or CPAngleCheck");
+    if (!p->have_cpd) return fail(ITCPD_E_STATE, "no CPD set");
+    if (c->has_prev && !p->prev_valid)
+        return fail(ITCPD_E_STATE, "check holds a previous CPD that this plan does not have");
+    DeviceGuard dg(p->device);
+    int rc;
+    if ((rc = ensure_prevcp(p))) return rc;
+    DevCheck* h = p->chk_host;
+    std::memset(h, 0, sizeof(DevCheck));
+    h->kind = c->kind;
+    h->iter = c->iter;
+    h->counter = c->counter;
+    h->max_counter = c->max_counter;
+    h->total_iter = c->total_iter;
+    h->tolerance = c->tolerance;
+    h->lastfit = c->lastfit;
+    h->final_fit = c->final_fit;
+    h->norm_prev = c->ref_norm;
+    h->has_prev = c->has_prev ? 1 : 0;
+    p->check_kind = c->kind;
+    ITCPD_HIP(hipMemcpyAsync(p->chk, h, sizeof(DevCheck), hipMemcpyHostToDevice, p->stream));
+    rc = launch_cp_check(p);
+    p->check_kind = 0;
+    if (rc) return rc;
+    ITCPD_HIP(hipMemcpyAsync(h, p->chk, sizeof(DevCheck), hipMemcpyDeviceToHost, p->stream));
+    ITCPD_HIP(hipStreamSynchronize(p->stream));
+    c->ref_norm = h->norm_prev;
+    c->has_prev = h->has_prev;
+    p->prev_valid = h->has_prev != 0;
+    c->iter = h->iter;
+    c->counter = h->counter;
+    c->total_iter = h->total_iter;
+    c->lastfit = h->lastfit;
+    c->final_fit = h->final_fit;
+    c->sweeps_run = 1;
+    c->converged = (h->done && !h->nan_flag) ? 1 : 0;
+    if (h->nan_flag) return fail(ITCPD_E_NAN, "Error NAN");
     return ITCPD_OK;
 }
 

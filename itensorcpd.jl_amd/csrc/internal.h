@@ -130,6 +130,13 @@ struct itcpd_plan {
     double* xg_part = nullptr;
     int xg_chunks = 0;
     bool prev_valid = false;
+    // leverage-score-sampled steps: grow-only workspaces, and the V the factorisation
+    // takes instead of the Hadamard of the Grams (when set)
+    double* smp_ws = nullptr;
+    size_t smp_ws_cap = 0;
+    int* smp_iws = nullptr;
+    size_t smp_iws_cap = 0;
+    const double* v_override = nullptr;
 };
 
 namespace itcpd {
@@ -173,6 +180,10 @@ int launch_random_init(itcpd_plan* p, uint64_t seed);
 int launch_fit_only(itcpd_plan* p, const double* Mlast, double ref_norm);
 int launch_hadamard(itcpd_plan* p, int mode, double* V);
 int launch_col_sumsq(itcpd_plan* p, int mode, const double* X);
+// leverage-score-sampled ALS (sampled.hip)
+int launch_leverage_scores(itcpd_plan* p, int mode, double* probs_dev, double** probs_used);
+int sampled_ws_prepare(itcpd_plan* p, int mode, int64_t ns);
+int launch_sampled_solve(itcpd_plan* p, int mode, int64_t ns, const int* samples, bool normal);
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

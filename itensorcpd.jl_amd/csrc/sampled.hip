@@ -1,0 +1,460 @@
+// Leverage-score-sampled ALS (LevScoreSampled, krp_lev_score_sampled.jl:9-57): the
+// device side of one projected least-squares step, and the leverage-score
+// probabilities that drive the sampling (math_tools/probability.jl:3-10).  The
+// sampling itself (StatsBase.sample with Weights in the reference) is host logic.
+//
+// One step for mode n with ns sampled index tuples of the other modes:
+//   K_s(s, r) = prod_q A_q(samples(s, q), r)            pivot_hadamard, had_contract.jl:277-295
+//   B(s, i)   = T(i at mode n, samples(s, :))           fused_flatten_sample, pivot_mapping.jl:90-118
+//                                                       (stored transposed: ns x I_n)
+//   normal:  X = ldiv_solve(K_s^T K_s, (B^T K_s)^T)^T   ProjectionAlgorithm.jl:56-58
+//   else:    X = (qr(K_s, ColumnNorm()) \ B)^T          ProjectionAlgorithm.jl:60
+// then row_norm into the plan's CPD.  The normal form reuses the plan's pivoted
+// Cholesky / QR-fallback factorisation and row solves with V = K_s^T K_s; the QR
+// form runs LAPACK dgelsy's steps (column-pivoted Householder QR, rank by
+// incremental condition estimation at rcond = min(m, n) eps, RZ reduction of the
+// trapezoid, minimum-norm solution) in one workgroup.
+#include "dense_la.h"
+#include "internal.h"
+
+namespace itcpd {
+
+namespace {
+
+constexpr int SNT = 512;   // threads of the one-workgroup QR kernels
+constexpr double EPS64 = 2.220446049250313e-16;
+
+struct OtherModes {
+    const double* A[ITCPD_MAX_ORDER];   // column-major factors of the other modes (ld = dim)
+    int64_t dim[ITCPD_MAX_ORDER];
+    int64_t tstride[ITCPD_MAX_ORDER];   // element stride of that mode in T
+    int n;
+};
+
+// K_s, column-major ns x R.
+__global__ __launch_bounds__(256) void k_pivot_hadamard(OtherModes om, const int* __restrict__ smp,
+                                                        int64_t ns, int R, double* __restrict__ Ks,
+                                                        const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= ns * R) return;
+    const int r = (int)(idx / ns);
+    const int64_t s = idx - (int64_t)r * ns;
+    double v = 1.0;
+    for (int q = 0; q < om.n; ++q) v *= om.A[q][smp[s + ns * q] + om.dim[q] * r];
+    Ks[idx] = v;
+}
+
+// B(s, i) = T(sampled fiber s, element i of mode n), column-major ns x In.
+template <typename TT>
+__global__ __launch_bounds__(256) void k_sample_fibers(const TT* __restrict__ T, OtherModes om,
+                                                       int64_t mstride, int64_t In,
+                                                       const int* __restrict__ smp, int64_t ns,
+                                                       double* __restrict__ B,
+                                                       const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= ns * In) return;
+    const int64_t i = idx / ns;
+    const int64_t s = idx - i * ns;
+    int64_t off = i * mstride;
+    for (int q = 0; q < om.n; ++q) off += (int64_t)smp[s + ns * q] * om.tstride[q];
+    B[idx] = (double)T[off];
+}
+
+// C(i, j) = sum_s A(s, i) B(s, j) (A: ns x m, B: ns x n, both ld ns; C: m x n), one
+// wave per output element, lanes over s.
+__global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, const double* __restrict__ B,
+                                                 int64_t ns, int m, int n, double* __restrict__ C,
+                                                 const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    const int64_t w = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= (int64_t)m * n) return;
+    const int i = (int)(w % m), j = (int)(w / m);
+    const double* a = A + ns * i;
+    const double* b = B + ns * j;
+    double s = 0.0;
+    for (int64_t k = lane; k < ns; k += 64) s = fma(a[k], b[k], s);
+    s = wave_sum(s);
+    if (lane == 0) C[i + (int64_t)m * j] = s;
+}
+
+// Householder reflector of x = (x[0], x[1 .. len)) (xLARFG): x[0] <- beta,
+// x[1 .. len) <- v (v[0] = 1 implicit); returns tau (0 when the tail is zero).
+template <int NT>
+__device__ double house(double* x, int len, double* redv) {
+    const int t = threadIdx.x;
+    double xs = 0.0;
+    for (int k = 1 + t; k < len; k += NT) xs += x[k] * x[k];
+    const double xnorm2 = block_sum<NT>(xs, redv);
+    const double alpha = x[0];
+    double tau = 0.0, beta = alpha, scal = 0.0;
+    if (xnorm2 > 0.0) {
+        beta = -copysign(hypot(alpha, sqrt(xnorm2)), alpha);
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+    }
+    __syncthreads();
+    if (xnorm2 > 0.0)
+        for (int k = 1 + t; k < len; k += NT) x[k] *= scal;
+    if (t == 0) x[0] = beta;
+    __syncthreads();
+    return tau;
+}
+
+// C <- (I - tau u u^T) C for the ncols columns of C (rows 0 .. len-1, ld ldc), with
+// u = (1, v[1 .. len)); one wave per column.
+template <int NT>
+__device__ void house_left(const double* v, int len, double tau, double* C, int64_t ldc, int64_t ncols) {
+    if (tau != 0.0) {
+        const int t = threadIdx.x, ln = t & 63;
+        for (int64_t j = t >> 6; j < ncols; j += NT / 64) {
+            double* c = C + ldc * j;
+            double s = 0.0;
+            for (int k = ln; k < len; k += 64) s += (k == 0 ? 1.0 : v[k]) * c[k];
+            const double f = tau * wave_sum(s);
+            for (int k = ln; k < len; k += 64) c[k] -= f * (k == 0 ? 1.0 : v[k]);
+        }
+    }
+    __syncthreads();
+}
+
+// Householder QR of W (m x n, ld) with column pivoting (xGEQP3 / xLAQP2: largest
+// remaining partial column norm first, norms downdated and recomputed when the
+// downdate loses accuracy) or without.  tau[min(m, n)], jpvt[n].
+template <int NT>
+__device__ void geqp3(double* W, int m, int n, int64_t ld, bool pivot, double* tau, int* jpvt,
+                      double* vn1, double* vn2, double* redv, int* redi) {
+    const int t = threadIdx.x;
+    for (int j = t; j < n; j += NT) jpvt[j] = j;
+    if (pivot) {
+        for (int j = t >> 6; j < n; j += NT / 64) {
+            const int ln = t & 63;
+            double ss = 0.0;
+            for (int k = ln; k < m; k += 64) ss += W[k + ld * j] * W[k + ld * j];
+            ss = wave_sum(ss);
+            if (ln == 0) { vn1[j] = sqrt(ss); vn2[j] = sqrt(ss); }
+        }
+    }
+    __syncthreads();
+    const double tol3z = 1.0536712127723509e-08;   // sqrt(eps)
+    const int kmax = m < n ? m : n;
+    for (int i = 0; i < kmax; ++i) {
+        if (pivot) {
+            const int p = block_argmax<NT>(i, n, [&](int j) { return fabs(vn1[j]); }, redv, redi);
+            const int pvt = (p < n) ? p : i;
+            if (pvt != i) {
+                for (int k = t; k < m; k += NT) {
+                    const double a = W[k + ld * pvt];
+                    W[k + ld * pvt] = W[k + ld * i];
+                    W[k + ld * i] = a;
+                }
+                if (t == 0) {
+                    const int q = jpvt[pvt]; jpvt[pvt] = jpvt[i]; jpvt[i] = q;
+                    vn1[pvt] = vn1[i];
+                    vn2[pvt] = vn2[i];
+                }
+            }
+            __syncthreads();
+        }
+        double* col = W + i + ld * i;
+        const double ti = house<NT>(col, m - i, redv);
+        if (t == 0) tau[i] = ti;
+        if (i + 1 < n) house_left<NT>(col, m - i, ti, col + ld, ld, n - i - 1);
+        if (pivot) {
+            for (int j = i + 1 + (t >> 6); j < n; j += NT / 64) {
+                const int ln = t & 63;
+                const double v1 = vn1[j];
+                if (v1 != 0.0) {
+                    double temp = fabs(W[i + ld * j]) / v1;
+                    temp = fmax(1.0 - temp * temp, 0.0);
+                    const double q = v1 / vn2[j];
+                    if (temp * q * q <= tol3z) {
+                        double ss = 0.0;
+                        for (int k = i + 1 + ln; k < m; k += 64) ss += W[k + ld * j] * W[k + ld * j];
+                        ss = wave_sum(ss);
+                        if (ln == 0) { vn1[j] = sqrt(ss); vn2[j] = vn1[j]; }
+                    } else if (ln == 0) {
+                        vn1[j] = v1 * sqrt(temp);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// qr(K, ColumnNorm()) \ B as LAPACK dgelsy: K (m x n) in W (overwritten), B (m x nrhs,
+// ld m) overwritten by Q^T B; the solution X(c, jpvt(k)) = y_c(k) goes to X (nrhs x n,
+// column-major, ld nrhs) through the scratch Y (n x nrhs).  info[0] = rank.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_lsq_qr(double* __restrict__ W, int m, int n, double* __restrict__ B,
+                                               int64_t nrhs, double* __restrict__ Y, double* __restrict__ X,
+                                               double* __restrict__ tau, double* __restrict__ ztau,
+                                               double* __restrict__ vwork, int* __restrict__ jpvt,
+                                               int* __restrict__ info, const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    __shared__ double redv[NT];
+    __shared__ int redi[NT];
+    __shared__ int s_rank;
+    const int t = threadIdx.x;
+    const int64_t ld = m;
+    const int kq = m < n ? m : n;
+    double* vn1 = vwork;
+    double* vn2 = vwork + n;
+    geqp3<NT>(W, m, n, ld, true, tau, jpvt, vn1, vn2, redv, redi);
+
+    // numerical rank: incremental condition estimation on R, rcond = min(m, n) eps
+    if (t == 0) {
+        const double rcond = (double)kq * EPS64;
+        double* xmin = vwork;
+        double* xmax = vwork + n;
+        double smax = fabs(W[0]);
+        double smin = smax;
+        int rk = 0;
+        if (kq > 0 && smax != 0.0) {
+            rk = 1;
+            xmin[0] = 1.0;
+            xmax[0] = 1.0;
+            while (rk < kq) {
+                const int i = rk;
+                double amin = 0.0, amax = 0.0;
+                for (int k = 0; k < i; ++k) {
+                    amin += xmin[k] * W[k + ld * i];
+                    amax += xmax[k] * W[k + ld * i];
+                }
+                const double gamma = W[i + ld * i];
+                double sminpr, s1, c1, smaxpr, s2, c2;
+                laic1(2, amin, smin, gamma, &sminpr, &s1, &c1);
+                laic1(1, amax, smax, gamma, &smaxpr, &s2, &c2);
+                if (smaxpr * rcond > sminpr) break;
+                for (int k = 0; k < i; ++k) { xmin[k] *= s1; xmax[k] *= s2; }
+                xmin[i] = c1;
+                xmax[i] = c2;
+                smin = sminpr;
+                smax = smaxpr;
+                ++rk;
+            }
+        }
+        s_rank = rk;
+        info[0] = rk;
+    }
+    __syncthreads();
+    const int rk = s_rank;
+
+    // RZ factorisation of the rk x n trapezoid [R11 R12] (xTZRZF / xLATRZ)
+    if (rk > 0 && rk < n) {
+        const int L = n - rk;
+        for (int i = rk - 1; i >= 0; --i) {
+            double xs = 0.0;
+            for (int q = t; q < L; q += NT) xs += W[i + ld * (rk + q)] * W[i + ld * (rk + q)];
+            const double xnorm2 = block_sum<NT>(xs, redv);
+            const double alpha = W[i + ld * i];
+            double tz = 0.0, beta = alpha, scal = 0.0;
+            if (xnorm2 > 0.0) {
+                beta = -copysign(hypot(alpha, sqrt(xnorm2)), alpha);
+                tz = (beta - alpha) / beta;
+                scal = 1.0 / (alpha - beta);
+            }
+            __syncthreads();
+            if (xnorm2 > 0.0)
+                for (int q = t; q < L; q += NT) W[i + ld * (rk + q)] *= scal;
+            if (t == 0) { W[i + ld * i] = beta; ztau[i] = tz; }
+            __syncthreads();
+            if (tz != 0.0) {
+                for (int k = t >> 6; k < i; k += NT / 64) {
+                    const int ln = t & 63;
+                    double sdot = (ln == 0) ? W[k + ld * i] : 0.0;
+                    for (int q = ln; q < L; q += 64) sdot += W[k + ld * (rk + q)] * W[i + ld * (rk + q)];
+                    const double f = tz * wave_sum(sdot);
+                    if (ln == 0) W[k + ld * i] -= f;
+                    for (int q = ln; q < L; q += 64) W[k + ld * (rk + q)] -= f * W[i + ld * (rk + q)];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // B <- Q^T B (the kq reflectors, in order)
+    for (int i = 0; i < kq; ++i) house_left<NT>(W + i + ld * i, m - i, tau[i], B + i, ld, nrhs);
+
+    // per right-hand side c: y = T11^-1 (Q^T b)(0:rk), zero beyond, y <- Z^T y, x(jpvt) = y
+    for (int64_t c = t; c < nrhs; c += NT) {
+        double* y = Y + (int64_t)n * c;
+        const double* b = B + ld * c;
+        for (int k = rk; k < n; ++k) y[k] = 0.0;
+        for (int j = rk - 1; j >= 0; --j) {
+            double s = b[j];
+            for (int k = j + 1; k < rk; ++k) s -= W[j + ld * k] * y[k];
+            y[j] = s / W[j + ld * j];
+        }
+        if (rk < n) {
+            for (int tq = 0; tq < rk; ++tq) {
+                const double tz = ztau[tq];
+                if (tz == 0.0) continue;
+                double part = y[tq];
+                for (int k = rk; k < n; ++k) part += W[tq + ld * k] * y[k];
+                const double f = tz * part;
+                y[tq] -= f;
+                for (int k = rk; k < n; ++k) y[k] -= f * W[tq + ld * k];
+            }
+        }
+        for (int k = 0; k < n; ++k) X[c + nrhs * jpvt[k]] = y[k];
+    }
+}
+
+// compute_leverage_score_probabilitiy (probability.jl:3-10): Q = the thin Q of the
+// unpivoted Householder QR of A (m x n), p(i) = sum_j Q(i, j)^2 / min(m, n).  W is
+// an m x n scratch (ld m); Q is formed in place as LAPACK xORG2R does.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_lev_scores(const double* __restrict__ A, int m, int n,
+                                                   double* __restrict__ W, double* __restrict__ tau,
+                                                   double* __restrict__ vwork, int* __restrict__ jpvt,
+                                                   double* __restrict__ probs,
+                                                   const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    __shared__ double redv[NT];
+    __shared__ int redi[NT];
+    const int t = threadIdx.x;
+    const int64_t ld = m;
+    for (int64_t x = t; x < (int64_t)m * n; x += NT) W[x] = A[x];
+    __syncthreads();
+    geqp3<NT>(W, m, n, ld, false, tau, jpvt, vwork, vwork + n, redv, redi);
+    const int kq = m < n ? m : n;
+    for (int i = kq - 1; i >= 0; --i) {
+        if (i < kq - 1) house_left<NT>(W + i + ld * i, m - i, tau[i], W + i + ld * (i + 1), ld, kq - 1 - i);
+        const double ti = tau[i];
+        for (int k = i + 1 + t; k < m; k += NT) W[k + ld * i] *= -ti;
+        for (int k = t; k < i; k += NT) W[k + ld * i] = 0.0;
+        if (t == 0) W[i + ld * i] = 1.0 - ti;
+        __syncthreads();
+    }
+    for (int i = t; i < m; i += NT) {
+        double s = 0.0;
+        for (int j = 0; j < kq; ++j) s += W[i + ld * j] * W[i + ld * j];
+        probs[i] = s / (double)kq;
+    }
+}
+
+OtherModes other_modes(const itcpd_plan* p, int mode) {
+    OtherModes om{};
+    int64_t stride = 1;
+    int q = 0;
+    for (int m = 0; m < p->N; ++m) {
+        if (m != mode) {
+            om.A[q] = p->A[m];
+            om.dim[q] = p->dims[m];
+            om.tstride[q] = stride;
+            ++q;
+        }
+        stride *= p->dims[m];
+    }
+    om.n = q;
+    return om;
+}
+
+int64_t tensor_stride(const itcpd_plan* p, int mode) {
+    int64_t s = 1;
+    for (int m = 0; m < mode; ++m) s *= p->dims[m];
+    return s;
+}
+
+// grow-only workspaces of the sampled path
+int ensure_sampled_ws(itcpd_plan* p, size_t doubles, size_t ints) {
+    int rc;
+    if (p->smp_ws_cap < doubles) {
+        ITCPD_HIP(hipStreamSynchronize(p->stream));
+        dfree(p->smp_ws);
+        p->smp_ws = nullptr;
+        p->smp_ws_cap = 0;
+        if ((rc = dmalloc(&p->smp_ws, doubles))) return rc;
+        p->smp_ws_cap = doubles;
+    }
+    if (p->smp_iws_cap < ints) {
+        ITCPD_HIP(hipStreamSynchronize(p->stream));
+        dfree(p->smp_iws);
+        p->smp_iws = nullptr;
+        p->smp_iws_cap = 0;
+        if ((rc = dmalloc(&p->smp_iws, ints))) return rc;
+        p->smp_iws_cap = ints;
+    }
+    return ITCPD_OK;
+}
+
+}  // namespace
+
+// probs_dev: a device buffer of I_n doubles, or null for the workspace's (returned in
+// *probs_used; valid until the next sampled-path call)
+int launch_leverage_scores(itcpd_plan* p, int mode, double* probs_dev, double** probs_used) {
+    const int m = (int)p->dims[mode], n = (int)p->R;
+    int rc;
+    if ((rc = ensure_sampled_ws(p, (size_t)m * n + 3 * (size_t)n + (size_t)m, (size_t)n + 4))) return rc;
+    double* W = p->smp_ws;
+    double* tau = W + (size_t)m * n;
+    double* vwork = tau + n;
+    double* probs = probs_dev ? probs_dev : vwork + 2 * n;
+    *probs_used = probs;
+    hipLaunchKernelGGL(k_lev_scores<SNT>, dim3(1), dim3(SNT), 0, p->stream, p->A[mode], m, n, W, tau, vwork,
+                       p->smp_iws, probs, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+// samples: device, ns x (N-1) column-major.  X (I_n x R) is left in p->X; info[0] =
+// rank of K_s (QR form) or the factorisation's finfo (normal form) is copied by the caller.
+int launch_sampled_solve(itcpd_plan* p, int mode, int64_t ns, const int* samples, bool normal) {
+    const int R = (int)p->R;
+    const int64_t In = p->dims[mode];
+    const OtherModes om = other_modes(p, mode);
+    double* Ks = p->smp_ws;                       // ns x R
+    double* B = Ks + (size_t)ns * R;              // ns x In
+    double* W = B + (size_t)ns * In;              // ns x R (QR form)
+    double* Y = W + (size_t)ns * R;               // R x In
+    double* Vs = Y + (size_t)R * In;              // R x R
+    double* tau = Vs + (size_t)R * R;             // R
+    double* ztau = tau + R;                       // R
+    double* vwork = ztau + R;                     // 2 R
+    int* jpvt = p->smp_iws + (size_t)ns * (p->N - 1);
+    int* info = jpvt + R;
+    const int64_t nk = ns * R, nb = ns * In;
+    hipLaunchKernelGGL(k_pivot_hadamard, dim3((unsigned)cdiv(nk, 256)), dim3(256), 0, p->stream, om, samples,
+                       ns, R, Ks, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    const int64_t ms = tensor_stride(p, mode);
+    if (p->elt_bytes == 8)
+        hipLaunchKernelGGL(k_sample_fibers<double>, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, p->stream,
+                           (const double*)p->T, om, ms, In, samples, ns, B, p->chk);
+    else
+        hipLaunchKernelGGL(k_sample_fibers<float>, dim3((unsigned)cdiv(nb, 256)), dim3(256), 0, p->stream,
+                           (const float*)p->T, om, ms, In, samples, ns, B, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    if (normal) {
+        // V = K_s^T K_s, M = B^T K_s, then the plan's factorisation and row solves on V
+        hipLaunchKernelGGL(k_gemm_tn, dim3((unsigned)cdiv((int64_t)R * R * 64, 256)), dim3(256), 0, p->stream,
+                           Ks, Ks, ns, R, R, Vs, p->chk);
+        ITCPD_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_gemm_tn, dim3((unsigned)cdiv(In * R * 64, 256)), dim3(256), 0, p->stream, B, Ks,
+                           ns, (int)In, R, p->M, p->chk);
+        ITCPD_HIP(hipGetLastError());
+        p->v_override = Vs;
+        int rc = launch_factor(p, mode);
+        p->v_override = nullptr;
+        if (rc) return rc;
+        return launch_solve(p, mode, p->M, 1, In * R, false, false);
+    }
+    ITCPD_HIP(hipMemcpyAsync(W, Ks, (size_t)nk * 8, hipMemcpyDeviceToDevice, p->stream));
+    hipLaunchKernelGGL(k_lsq_qr<SNT>, dim3(1), dim3(SNT), 0, p->stream, W, (int)ns, R, B, In, Y, p->X, tau,
+                       ztau, vwork, jpvt, info, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
+}
+
+int sampled_ws_prepare(itcpd_plan* p, int mode, int64_t ns) {
+    const int64_t R = p->R, In = p->dims[mode];
+    const size_t doubles = (size_t)(2 * ns * R + ns * In + R * In + R * R + 4 * R);
+    const size_t ints = (size_t)(ns * (p->N - 1) + R + 4);
+    return ensure_sampled_ws(p, doubles, ints);
+}
+
+}  // namespace itcpd

@@ -1041,24 +1041,34 @@ int launch_cp_check(itcpd_plan* p) {
     return ITCPD_OK;
 }
 
-int launch_factor_on(itcpd_plan* p, int mode, hipStream_t st) {
+int launch_factor_on(itcpd_plan* p, int mode_in, hipStream_t st) {
     const int R = (int)p->R;
+    // a set V override: the factorisation of that matrix (as the Hadamard of {V}
+    // with mode 1 skipped), else of the Hadamard of the other Grams
+    GramSet gs = grams_of(p);
+    int mode = mode_in;
+    if (p->v_override) {
+        gs.G[0] = p->v_override;
+        gs.G[1] = p->v_override;
+        gs.N = 2;
+        mode = 1;
+    }
     double* W = p->qr_aux;
     double* qtau = W + (size_t)R * R;
     double* ztau = qtau + R;
     double* vwork = ztau + R;
     if (R <= 64) {
         hipLaunchKernelGGL(k_factor64, dim3(1), dim3(256), FACTOR64_LDS,
-                           st, grams_of(p), mode, R, 1e-6, p->U, p->Upad, p->invdiag, p->piv,
+                           st, gs, mode, R, 1e-6, p->U, p->Upad, p->invdiag, p->piv,
                            p->finfo, W, qtau, ztau, p->piv + R, vwork, p->chk);
     } else if (R <= 128) {
-        hipLaunchKernelGGL(k_factor128, dim3(1), dim3(1024), F128_LDS, st, grams_of(p), mode, R, 1e-6,
+        hipLaunchKernelGGL(k_factor128, dim3(1), dim3(1024), F128_LDS, st, gs, mode, R, 1e-6,
                            p->U, p->invdiag, p->piv, p->finfo, W, qtau, ztau, p->piv + R, vwork, p->chk);
     } else {
         // right-looking, all threads; V in LDS up to R = 128, else in the U buffer
         const bool lds = R <= 128;
         hipLaunchKernelGGL(k_factor, dim3(1), dim3(FACTOR_THREADS), lds ? (size_t)R * R * 8 : 0, st,
-                           grams_of(p), mode, R, 1e-6, p->U, p->invdiag, p->piv, p->finfo, W, qtau,
+                           gs, mode, R, 1e-6, p->U, p->invdiag, p->piv, p->finfo, W, qtau,
                            ztau, p->piv + R, vwork, lds ? 1 : 0, p->chk);
     }
     ITCPD_HIP(hipGetLastError());

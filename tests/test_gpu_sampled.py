@@ -1,0 +1,114 @@
+"""Leverage-score-sampled ALS (LevScoreSampled) on the GPU against the oracle
+(oracle/sampled.py): the leverage scores, one projected least-squares step with the
+same samples (normal-equation and QR forms; tall and wide samples; the R > 64
+factorisation; an FP32-stored tensor), whole runs with the same draws, and the
+reference's own bar (test/rand_cp_als.jl:104-131)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import oracle.sampled as S
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("dims,R", [((30, 20, 7), 6), ((12, 9, 7), 10), ((40, 6, 5, 4), 3)])
+def test_leverage_scores_match_oracle(itcpd, dims, R):
+    rng = np.random.default_rng(1)
+    T = rng.standard_normal(dims)
+    init = O.random_CPD(T, R, np.random.default_rng(2))
+    with itcpd.ALSPlan(T, R) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        for n in range(len(dims)):
+            got = plan.leverage_scores(n)
+            ref = S.leverage_score_probability(init.factors[n])
+            assert np.abs(got - ref).max() < 1e-13, n
+
+
+STEP_CASES = [((12, 13, 3), 5, 100, np.float64), ((9, 8, 7), 6, 3, np.float64),
+              ((20, 18, 16, 5), 80, 400, np.float64), ((12, 13, 3), 5, 60, np.float32)]
+
+
+@pytest.mark.parametrize("normal", [True, False])
+@pytest.mark.parametrize("dims,R,ns,dt", STEP_CASES,
+                         ids=[f"{'x'.join(map(str, d))}-R{r}-ns{n}-{np.dtype(t).name}" for d, r, n, t in STEP_CASES])
+def test_sampled_step_matches_oracle(itcpd, dims, R, ns, dt, normal):
+    """Same samples on both sides; each mode in turn, so later modes see the updated
+    factors.  Wide samples (ns < R) exercise the normal form's QR fallback and the QR
+    form's minimum-norm solve."""
+    rng = np.random.default_rng(3)
+    T = rng.standard_normal(dims).astype(dt)
+    T64 = T.astype(np.float64)
+    init = O.random_CPD(T64, R, np.random.default_rng(4))
+    fac = [A.copy() for A in init.factors]
+    with itcpd.ALSPlan(T, R) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        for n in range(len(dims)):
+            w = [S.leverage_score_probability(A) for A in fac]
+            smp = S.sample_factor_matrices(ns, n, w, np.random.default_rng(10 + n))
+            plan.sampled_solve(n, smp, normal)
+            fac[n], lam = S.sampled_solve(T64, fac, n, smp, normal)
+            got = plan.get_cpd()
+            tol = 1e-8 if ns < R else 1e-10
+            assert rel(got.factors[n], fac[n]) < tol, n
+            assert rel(got.lam, lam) < tol, n
+
+
+@pytest.mark.parametrize("normal", [False, True])
+@pytest.mark.parametrize("check", ["diff", "angle", "fit"])
+def test_sampled_als_matches_oracle(itcpd, normal, check):
+    """A whole run, the draws from the same seed: the device's leverage scores match
+    the oracle's to rounding, so both sides draw the same samples every step."""
+    cpT = O.random_CPD(np.zeros((12, 13, 3)), 5, np.random.default_rng(101))
+    T = O.reconstruct(cpT.factors, cpT.lam)
+    cp0 = O.random_CPD(T, 5, np.random.default_rng(201))
+    mk = {"diff": (itcpd.CPDiffCheck, O.CPDiffCheck), "angle": (itcpd.CPAngleCheck, O.CPAngleCheck),
+          "fit": (lambda t, m: itcpd.FitCheck(t, m, 1.0), lambda t, m: O.FitCheck(t, m, 1.0))}[check]
+    cg, cc = mk[0](1e-5, 10), mk[1](1e-5, 10)
+    got = itcpd.als_optimize(T, itcpd.CPD(cp0.factors, cp0.lam), alg=itcpd.LevScoreSampled(100), check=cg,
+                             normal=normal, rng=301)
+    ref = S.optimize_sampled(T, cp0, 100, check=cc, normal=normal, rng=301)
+    for a, b in zip(got.factors, ref.factors):
+        assert rel(a, b) < 1e-8
+    assert rel(got.lam, ref.lam) < 1e-8
+    assert cg.iter == cc.iter
+
+
+def test_stop_resample_zero_matches_oracle(itcpd):
+    rng = np.random.default_rng(7)
+    T = rng.standard_normal((6, 5, 4))
+    cp0 = O.random_CPD(T, 3, np.random.default_rng(8))
+    got = itcpd.als_optimize(T, itcpd.CPD(cp0.factors, cp0.lam), alg=itcpd.LevScoreSampled(30),
+                             check=itcpd.NoCheck(3), normal=True, stop_resample=0, rng=9)
+    ref = S.optimize_sampled(T, cp0, 30, check=O.NoCheck(3), normal=True, stop_resample=0, rng=9)
+    for a, b in zip(got.factors, ref.factors):
+        assert rel(a, b) < 1e-10
+
+
+@pytest.mark.parametrize("normal", [False, True])
+def test_reference_bar(itcpd, normal):
+    """rand_cp_als.jl:104-131: best of 3 tries within relative error 0.1."""
+    cpT = O.random_CPD(np.zeros((12, 13, 3)), 5, np.random.default_rng(101))
+    T = O.reconstruct(cpT.factors, cpT.lam)
+    cp0 = itcpd.CPD(*[getattr(O.random_CPD(T, 5, np.random.default_rng(201)), k) for k in ("factors", "lam")])
+    errs = []
+    for tr in range(3):
+        out = itcpd.als_optimize(T, cp0, alg=itcpd.LevScoreSampled(100), check=itcpd.CPAngleCheck(1e-5, 10),
+                                 normal=normal, rng=301 + tr)
+        errs.append(rel(O.reconstruct(out.factors, out.lam), T))
+    assert min(errs) < 0.1
+
+
+def test_out_of_range_sample_is_rejected(itcpd):
+    T = np.random.default_rng(0).standard_normal((5, 4, 3))
+    init = O.random_CPD(T, 2, np.random.default_rng(1))
+    with itcpd.ALSPlan(T, 2) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        with pytest.raises(itcpd.ITCPDError):
+            plan.sampled_solve(0, np.array([[0, 3]], dtype=np.int32))   # mode 2 has 3 indices
+        with pytest.raises(ValueError):
+            plan.sampled_solve(0, np.zeros((4, 3), dtype=np.int32))
