@@ -181,6 +181,7 @@ int itcpd_leverage_scores(itcpd_plan* plan, int32_t mode, double* probs, int32_t
  * then row_norm into the plan's CPD (A_n, lambda, G_n).
  * samples: nsamps x (N-1) column-major, 0-based indices of the other modes in mode
  * order (the reference's 1-based `sampled_cols` minus one); every index is checked.
+ * The QR form takes R <= 2048.
  * info (nullable, 2 ints): [0] = 1 when the normal form's Cholesky fell back to QR,
  * [1] = numerical rank of the solve. */
 int itcpd_sampled_solve(itcpd_plan* plan, int32_t mode, int64_t nsamps, const int32_t* samples,

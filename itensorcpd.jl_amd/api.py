@@ -487,6 +487,47 @@ class ALSPlan:
                                               L.ITCPD_MEM_HOST, 1 if normal else 0, info))
         return bool(info[0]), int(info[1])
 
+    def run_sampled(self, alg: "LevScoreSampled", check, normal: bool = False, stop_resample: int = -1,
+                    rng=None, verbose: bool = False) -> int:
+        """compute_als(::LevScoreSampled) (optimizers/als_optimizers/randomized/
+        krp_lev_score_sampled.jl:1-37) and optimize (optimize.jl:6-32) with project_krp /
+        post_solve (algorithms/.../krp_lev_score_sampled.jl:18-57) and the
+        ProjectionAlgorithm check (ProjectionAlgorithm.jl:20-53), from the plan's current
+        CPD.  The sampled KRP, sampled fibers, solves, leverage scores and CP checks run
+        on the device; the draws are made here, as the reference makes them on the host
+        (StatsBase).  Returns the number of sweeps run."""
+        rng = np.random.default_rng(rng)
+        N = len(self.dims)
+        weights = [self.leverage_scores(m) for m in range(N)]
+        samples = [_sample_factor_matrices(alg.nsamples(m), m, weights, rng) for m in range(N)]
+        it, sweeps = check.iter, 0
+        while it < check.max_counter:
+            for n in range(N):
+                if stop_resample < 0 or stop_resample > check.iter:
+                    samples[n] = _sample_factor_matrices(alg.nsamples(n), n, weights, rng)
+                self.sampled_solve(n, samples[n], normal)
+                weights[n] = self.leverage_scores(n)
+            sweeps += 1
+            if isinstance(check, FitCheck):
+                # a FitCheck only counts sweeps for these algorithms
+                check.iter += 1
+                if check.iter >= check.max_counter:
+                    check.iter = 0
+                stop = False
+            elif isinstance(check, NoCheck):
+                check.iter += 1
+                stop = check.iter == check.max_counter
+                if stop:
+                    check.iter = 0
+            else:
+                stop = self.check_cpd(check)
+            if verbose:
+                print(f"{self.rank}\t {check.iter}")
+            if stop:
+                break
+            it += 1
+        return sweeps
+
     def check_cpd(self, check) -> bool:
         """One evaluation of a CPDiffCheck / CPAngleCheck on the current CPD (the check's
         check_converge); True = stop."""
@@ -576,45 +617,12 @@ def _sample_factor_matrices(nsamps: int, skip: int, probs, rng) -> np.ndarray:
     return np.stack(cols, axis=1)
 
 
-def _optimize_sampled(target, cp: CPD, alg: LevScoreSampled, check, normal: bool, stop_resample: int,
+def _optimize_sampled(target, cp: CPD, alg: "LevScoreSampled", check, normal: bool, stop_resample: int,
                       rng, verbose: bool) -> CPD:
-    """compute_als(::LevScoreSampled) (optimizers/als_optimizers/randomized/
-    krp_lev_score_sampled.jl:1-37) and optimize (optimize.jl:6-32) with project_krp /
-    post_solve (algorithms/.../krp_lev_score_sampled.jl:18-57) and the
-    ProjectionAlgorithm check (ProjectionAlgorithm.jl:20-53).  The sampled KRP,
-    sampled fibers, solves, leverage scores and CP checks run on the device; the
-    draws are made here, as the reference makes them on the host (StatsBase)."""
-    rng = np.random.default_rng(rng)
-    N = len(cp.factors)
+    """compute_als(::LevScoreSampled) + optimize on a fresh plan (see ALSPlan.run_sampled)."""
     with ALSPlan(target, cp.rank) as plan:
         plan.set_cpd(cp)
-        weights = [plan.leverage_scores(m) for m in range(N)]
-        samples = [_sample_factor_matrices(alg.nsamples(m), m, weights, rng) for m in range(N)]
-        it = check.iter
-        while it < check.max_counter:
-            for n in range(N):
-                if stop_resample < 0 or stop_resample > check.iter:
-                    samples[n] = _sample_factor_matrices(alg.nsamples(n), n, weights, rng)
-                plan.sampled_solve(n, samples[n], normal)
-                weights[n] = plan.leverage_scores(n)
-            if isinstance(check, FitCheck):
-                # a FitCheck only counts sweeps for these algorithms
-                check.iter += 1
-                if check.iter >= check.max_counter:
-                    check.iter = 0
-                stop = False
-            elif isinstance(check, NoCheck):
-                check.iter += 1
-                stop = check.iter == check.max_counter
-                if stop:
-                    check.iter = 0
-            else:
-                stop = plan.check_cpd(check)
-            if verbose:
-                print(f"{cp.rank}\t {check.iter}")
-            if stop:
-                break
-            it += 1
+        plan.run_sampled(alg, check, normal=normal, stop_resample=stop_resample, rng=rng, verbose=verbose)
         return plan.get_cpd()
 
 

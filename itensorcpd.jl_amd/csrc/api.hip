@@ -658,6 +658,7 @@ int itcpd_sampled_solve(itcpd_plan* p, int32_t mode, int64_t nsamps, const int32
     if (p->N < 2) return fail(ITCPD_E_ARG, "sampling needs at least two modes");
     if (nsamps < 1 || nsamps > ((int64_t)1 << 31) / p->N) return fail(ITCPD_E_ARG, "nsamps out of range");
     if (!p->have_cpd) return fail(ITCPD_E_STATE, "no CPD set");
+    if (!normal && p->R > 2048) return fail(ITCPD_E_UNSUPPORTED, "the QR form takes rank <= 2048");
     DeviceGuard dg(p->device);
     int rc;
     if ((rc = reset_check_flags(p))) return rc;

@@ -185,15 +185,14 @@ __device__ void geqp3(double* W, int m, int n, int64_t ld, bool pivot, double* t
     }
 }
 
-// qr(K, ColumnNorm()) \ B as LAPACK dgelsy: K (m x n) in W (overwritten), B (m x nrhs,
-// ld m) overwritten by Q^T B; the solution X(c, jpvt(k)) = y_c(k) goes to X (nrhs x n,
-// column-major, ld nrhs) through the scratch Y (n x nrhs).  info[0] = rank.
+// qr(K, ColumnNorm()) \ B as LAPACK dgelsy, factor part (one workgroup): K (m x n) in
+// W is overwritten by the column-pivoted QR (tau, jpvt), the numerical rank goes to
+// info[0], and for rank < n the rk x n trapezoid is reduced to [T11 0] Z (ztau).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_lsq_qr(double* __restrict__ W, int m, int n, double* __restrict__ B,
-                                               int64_t nrhs, double* __restrict__ Y, double* __restrict__ X,
-                                               double* __restrict__ tau, double* __restrict__ ztau,
-                                               double* __restrict__ vwork, int* __restrict__ jpvt,
-                                               int* __restrict__ info, const DevCheck* __restrict__ chk) {
+__global__ __launch_bounds__(NT) void k_lsq_qr_factor(double* __restrict__ W, int m, int n,
+                                                      double* __restrict__ tau, double* __restrict__ ztau,
+                                                      double* __restrict__ vwork, int* __restrict__ jpvt,
+                                                      int* __restrict__ info, const DevCheck* __restrict__ chk) {
     if (chk->done) return;
     __shared__ double redv[NT];
     __shared__ int redi[NT];
@@ -275,33 +274,63 @@ __global__ __launch_bounds__(NT) void k_lsq_qr(double* __restrict__ W, int m, in
             __syncthreads();
         }
     }
+}
 
-    // B <- Q^T B (the kq reflectors, in order)
-    for (int i = 0; i < kq; ++i) house_left<NT>(W + i + ld * i, m - i, tau[i], B + i, ld, nrhs);
-
-    // per right-hand side c: y = T11^-1 (Q^T b)(0:rk), zero beyond, y <- Z^T y, x(jpvt) = y
-    for (int64_t c = t; c < nrhs; c += NT) {
-        double* y = Y + (int64_t)n * c;
-        const double* b = B + ld * c;
-        for (int k = rk; k < n; ++k) y[k] = 0.0;
-        for (int j = rk - 1; j >= 0; --j) {
-            double s = b[j];
-            for (int k = j + 1; k < rk; ++k) s -= W[j + ld * k] * y[k];
-            y[j] = s / W[j + ld * j];
-        }
-        if (rk < n) {
-            for (int tq = 0; tq < rk; ++tq) {
-                const double tz = ztau[tq];
-                if (tz == 0.0) continue;
-                double part = y[tq];
-                for (int k = rk; k < n; ++k) part += W[tq + ld * k] * y[k];
-                const double f = tz * part;
-                y[tq] -= f;
-                for (int k = rk; k < n; ++k) y[k] -= f * W[tq + ld * k];
-            }
-        }
-        for (int k = 0; k < n; ++k) X[c + nrhs * jpvt[k]] = y[k];
+// dgelsy's solution for one right-hand side per wave (column c of B, m x nrhs, ld m):
+// b <- Q^T b, y = T11^-1 b(0:rk), zero beyond rk, y <- Z^T y (rank < n), and
+// X(c, jpvt(k)) = y(k) (X: nrhs x n, ld nrhs).  Lane l owns the rows r = l (mod 64)
+// of b, so no lane reads another lane's store to global memory; y (n entries per
+// wave) lives in LDS, and every hand-over between lanes waits for the LDS writes.
+__global__ __launch_bounds__(256) void k_lsq_qr_apply(const double* __restrict__ W, int m, int n,
+                                                      double* __restrict__ B, int64_t nrhs,
+                                                      double* __restrict__ X,
+                                                      const double* __restrict__ tau,
+                                                      const double* __restrict__ ztau,
+                                                      const int* __restrict__ jpvt, const int* __restrict__ info,
+                                                      const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    extern __shared__ double ylds[];
+    const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * 4 + wave;
+    if (c >= nrhs) return;
+    const int64_t ld = m;
+    const int kq = m < n ? m : n;
+    const int rk = info[0];
+    double* b = B + ld * c;
+    for (int i = 0; i < kq; ++i) {
+        const double ti = tau[i];
+        if (ti == 0.0) continue;
+        double s = 0.0;
+        for (int r = ln; r < m; r += 64)
+            if (r >= i) s += (r == i ? 1.0 : W[r + ld * i]) * b[r];
+        const double f = ti * wave_sum(s);
+        for (int r = ln; r < m; r += 64)
+            if (r >= i) b[r] -= f * (r == i ? 1.0 : W[r + ld * i]);
     }
+    double* y = ylds + (size_t)wave * n;
+    for (int k = ln; k < n; k += 64) y[k] = (k < rk) ? b[k] : 0.0;
+    __threadfence_block();
+    for (int j = rk - 1; j >= 0; --j) {
+        double s = 0.0;
+        for (int k = j + 1 + ln; k < rk; k += 64) s += W[j + ld * k] * y[k];
+        s = wave_sum(s);
+        if (ln == 0) y[j] = (y[j] - s) / W[j + ld * j];
+        __threadfence_block();
+    }
+    if (rk < n) {
+        for (int tq = 0; tq < rk; ++tq) {
+            const double tz = ztau[tq];
+            if (tz == 0.0) continue;
+            double part = 0.0;
+            for (int k = rk + ln; k < n; k += 64) part += W[tq + ld * k] * y[k];
+            const double f = tz * (wave_sum(part) + y[tq]);
+            __threadfence_block();
+            if (ln == 0) y[tq] -= f;
+            for (int k = rk + ln; k < n; k += 64) y[k] -= f * W[tq + ld * k];
+            __threadfence_block();
+        }
+    }
+    for (int k = ln; k < n; k += 64) X[c + nrhs * jpvt[k]] = y[k];
 }
 
 // compute_leverage_score_probabilitiy (probability.jl:3-10): Q = the thin Q of the
@@ -410,8 +439,7 @@ int launch_sampled_solve(itcpd_plan* p, int mode, int64_t ns, const int* samples
     double* Ks = p->smp_ws;                       // ns x R
     double* B = Ks + (size_t)ns * R;              // ns x In
     double* W = B + (size_t)ns * In;              // ns x R (QR form)
-    double* Y = W + (size_t)ns * R;               // R x In
-    double* Vs = Y + (size_t)R * In;              // R x R
+    double* Vs = W + (size_t)ns * R;              // R x R
     double* tau = Vs + (size_t)R * R;             // R
     double* ztau = tau + R;                       // R
     double* vwork = ztau + R;                     // 2 R
@@ -444,15 +472,18 @@ int launch_sampled_solve(itcpd_plan* p, int mode, int64_t ns, const int* samples
         return launch_solve(p, mode, p->M, 1, In * R, false, false);
     }
     ITCPD_HIP(hipMemcpyAsync(W, Ks, (size_t)nk * 8, hipMemcpyDeviceToDevice, p->stream));
-    hipLaunchKernelGGL(k_lsq_qr<SNT>, dim3(1), dim3(SNT), 0, p->stream, W, (int)ns, R, B, In, Y, p->X, tau,
-                       ztau, vwork, jpvt, info, p->chk);
+    hipLaunchKernelGGL(k_lsq_qr_factor<SNT>, dim3(1), dim3(SNT), 0, p->stream, W, (int)ns, R, tau, ztau, vwork,
+                       jpvt, info, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_lsq_qr_apply, dim3((unsigned)cdiv(In, 4)), dim3(256), (size_t)4 * R * 8, p->stream, W,
+                       (int)ns, R, B, In, p->X, tau, ztau, jpvt, info, p->chk);
     ITCPD_HIP(hipGetLastError());
     return ITCPD_OK;
 }
 
 int sampled_ws_prepare(itcpd_plan* p, int mode, int64_t ns) {
     const int64_t R = p->R, In = p->dims[mode];
-    const size_t doubles = (size_t)(2 * ns * R + ns * In + R * In + R * R + 4 * R);
+    const size_t doubles = (size_t)(2 * ns * R + ns * In + R * R + 4 * R);
     const size_t ints = (size_t)(ns * (p->N - 1) + R + 4);
     return ensure_sampled_ws(p, doubles, ints);
 }
