@@ -100,22 +100,25 @@ __device__ __forceinline__ void mttkrp_task(const double* __restrict__ T, int I,
         if (n == 0) { base = idx + IJ * o; step = I; }                      // T(idx, q, o)
         else if (n == 1) { base = (int64_t)I * idx + IJ * o; step = 1; }    // T(q, idx, o)
         else { base = (int64_t)I * o + IJ * idx; step = 1; }                // T(q, o, idx)
+        // eight tensor loads issued before their use (the loop is latency bound); the
+        // products still add up in q order
         int q = 0;
-        for (; q + 2 <= innerN; q += 2) {
-            const double t0 = T[base + step * q];
-            const double t1 = T[base + step * (q + 1)];
-            const dv2* a0 = reinterpret_cast<const dv2*>(Ain + q * RB);
-            const dv2* a1 = reinterpret_cast<const dv2*>(Ain + (q + 1) * RB);
+        for (; q + 8 <= innerN; q += 8) {
+            double tq[8];
 #pragma unroll
-            for (int r2 = 0; r2 < RC / 2; ++r2) {
-                const dv2 x0 = a0[r2], x1 = a1[r2];
-                s[2 * r2] += t0 * x0.x;
-                s[2 * r2 + 1] += t0 * x0.y;
-                s[2 * r2] += t1 * x1.x;
-                s[2 * r2 + 1] += t1 * x1.y;
+            for (int u = 0; u < 8; ++u) tq[u] = T[base + step * (q + u)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const dv2* a = reinterpret_cast<const dv2*>(Ain + (q + u) * RB);
+#pragma unroll
+                for (int r2 = 0; r2 < RC / 2; ++r2) {
+                    const dv2 x = a[r2];
+                    s[2 * r2] += tq[u] * x.x;
+                    s[2 * r2 + 1] += tq[u] * x.y;
+                }
             }
         }
-        if (q < innerN) {
+        for (; q < innerN; ++q) {
             const double t0 = T[base + step * q];
             const dv2* a0 = reinterpret_cast<const dv2*>(Ain + q * RB);
 #pragma unroll
