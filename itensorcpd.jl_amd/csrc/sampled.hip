@@ -335,14 +335,15 @@ __global__ __launch_bounds__(256) void k_lsq_qr_apply(const double* __restrict__
 
 // compute_leverage_score_probabilitiy (probability.jl:3-10): Q = the thin Q of the
 // unpivoted Householder QR of A (m x n), p(i) = sum_j Q(i, j)^2 / min(m, n).  W is
-// an m x n scratch (ld m); Q is formed in place as LAPACK xORG2R does.
+// an m x n scratch (ld m); Q is formed in place as LAPACK xORG2R does.  Runs only
+// where the Gram form below cannot (need != 0, or need null).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_lev_scores(const double* __restrict__ A, int m, int n,
                                                    double* __restrict__ W, double* __restrict__ tau,
                                                    double* __restrict__ vwork, int* __restrict__ jpvt,
-                                                   double* __restrict__ probs,
+                                                   double* __restrict__ probs, const int* __restrict__ need,
                                                    const DevCheck* __restrict__ chk) {
-    if (chk->done) return;
+    if (chk->done || (need && *need == 0)) return;
     __shared__ double redv[NT];
     __shared__ int redi[NT];
     const int t = threadIdx.x;
@@ -364,6 +365,101 @@ __global__ __launch_bounds__(NT) void k_lev_scores(const double* __restrict__ A,
         for (int j = 0; j < kq; ++j) s += W[i + ld * j] * W[i + ld * j];
         probs[i] = s / (double)kq;
     }
+}
+
+// Leverage scores of a full-column-rank A (m > n) through its Gram G = A^T A:
+// sum_j Q(i, j)^2 = a_i^T G^-1 a_i, the diagonal of the projector onto range(A).
+// k_gram_chol_inv (one workgroup): G = U^T U (unpivoted Cholesky) and G^-1 =
+// U^-1 U^-T into Gi; need = 1 when a pivot falls below n eps max(diag G) (the
+// Householder kernel then runs instead).  G (n x n) is overwritten.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_gram_chol_inv(double* __restrict__ G, int n, double* __restrict__ Ui,
+                                                      double* __restrict__ Gi, int* __restrict__ need,
+                                                      const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    __shared__ double redv[NT];
+    __shared__ int s_bad;
+    const int t = threadIdx.x;
+    double dmax = 0.0;
+    for (int j = t; j < n; j += NT) dmax = fmax(dmax, G[j + (int64_t)n * j]);
+    redv[t] = dmax;
+    if (t == 0) s_bad = 0;
+    __syncthreads();
+    for (int o = NT / 2; o > 0; o >>= 1) {
+        if (t < o) redv[t] = fmax(redv[t], redv[t + o]);
+        __syncthreads();
+    }
+    const double thr = (double)n * EPS64 * redv[0];
+    // right-looking: row j of U in the upper triangle of G (column-major, G(j, k) at j + n k)
+    for (int j = 0; j < n; ++j) {
+        const double d = G[j + (int64_t)n * j];
+        if (!(d > thr)) {
+            if (t == 0) s_bad = 1;
+            __syncthreads();
+            break;
+        }
+        const double ujj = sqrt(d), inv = 1.0 / ujj;
+        __syncthreads();
+        for (int k = j + 1 + t; k < n; k += NT) G[j + (int64_t)n * k] *= inv;
+        if (t == 0) G[j + (int64_t)n * j] = ujj;
+        __syncthreads();
+        const int r = n - j - 1;
+        for (int x = t; x < r * r; x += NT) {
+            const int a = j + 1 + x % r, b = j + 1 + x / r;
+            if (a <= b) G[a + (int64_t)n * b] -= G[j + (int64_t)n * a] * G[j + (int64_t)n * b];
+        }
+        __syncthreads();
+    }
+    if (s_bad) {
+        if (t == 0) *need = 1;
+        return;
+    }
+    // Ui = U^-1 (upper), one column per thread: U x = e_c
+    for (int cidx = t; cidx < n; cidx += NT) {
+        for (int i = n - 1; i >= 0; --i) {
+            double v = (i == cidx) ? 1.0 : 0.0;
+            if (i <= cidx) {
+                for (int k = i + 1; k <= cidx; ++k) v -= G[i + (int64_t)n * k] * Ui[k + (int64_t)n * cidx];
+                v /= G[i + (int64_t)n * i];
+            }
+            Ui[i + (int64_t)n * cidx] = (i <= cidx) ? v : 0.0;
+        }
+    }
+    __syncthreads();
+    // Gi = Ui Ui^T
+    for (int x = t; x < n * n; x += NT) {
+        const int a = x % n, b = x / n;
+        const int k0 = a > b ? a : b;
+        double v = 0.0;
+        for (int k = k0; k < n; ++k) v += Ui[a + (int64_t)n * k] * Ui[b + (int64_t)n * k];
+        Gi[x] = v;
+    }
+    if (t == 0) *need = 0;
+}
+
+// p(i) = a_i^T Gi a_i / n, one thread per row (skipped when the Householder kernel runs)
+__global__ __launch_bounds__(256) void k_lev_rows(const double* __restrict__ A, int m, int n,
+                                                  const double* __restrict__ Gi, const int* __restrict__ need,
+                                                  double* __restrict__ probs, const DevCheck* __restrict__ chk) {
+    if (chk->done || *need != 0) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double aj = A[i + (int64_t)m * j];
+        double u = 0.0;
+        for (int k = 0; k < n; ++k) u += Gi[k + (int64_t)n * j] * A[i + (int64_t)m * k];
+        s += aj * u;
+    }
+    probs[i] = s / (double)n;
+}
+
+// p(i) = 1 / m for m <= n (Q is square and orthogonal: every row has norm 1)
+__global__ __launch_bounds__(256) void k_fill_value(double* __restrict__ out, int m, double v,
+                                                    const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < m) out[i] = v;
 }
 
 OtherModes other_modes(const itcpd_plan* p, int mode) {
@@ -418,14 +514,34 @@ int ensure_sampled_ws(itcpd_plan* p, size_t doubles, size_t ints) {
 int launch_leverage_scores(itcpd_plan* p, int mode, double* probs_dev, double** probs_used) {
     const int m = (int)p->dims[mode], n = (int)p->R;
     int rc;
-    if ((rc = ensure_sampled_ws(p, (size_t)m * n + 3 * (size_t)n + (size_t)m, (size_t)n + 4))) return rc;
+    // W (m n) | tau (n) | vwork (2n) | probs (m) | G, Ui, Gi (3 n^2)
+    if ((rc = ensure_sampled_ws(p, (size_t)m * n + 3 * (size_t)n + (size_t)m + 3 * (size_t)n * n, (size_t)n + 4)))
+        return rc;
     double* W = p->smp_ws;
     double* tau = W + (size_t)m * n;
     double* vwork = tau + n;
     double* probs = probs_dev ? probs_dev : vwork + 2 * n;
+    double* G = vwork + 2 * n + m;
+    double* Ui = G + (size_t)n * n;
+    double* Gi = Ui + (size_t)n * n;
+    int* need = p->smp_iws + n;
     *probs_used = probs;
+    if (m <= n) {
+        hipLaunchKernelGGL(k_fill_value, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, p->stream, probs, m,
+                           1.0 / (double)m, p->chk);
+        ITCPD_HIP(hipGetLastError());
+        return ITCPD_OK;
+    }
+    hipLaunchKernelGGL(k_gemm_tn, dim3((unsigned)cdiv((int64_t)n * n * 64, 256)), dim3(256), 0, p->stream,
+                       p->A[mode], p->A[mode], (int64_t)m, n, n, G, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_gram_chol_inv<256>, dim3(1), dim3(256), 0, p->stream, G, n, Ui, Gi, need, p->chk);
+    ITCPD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_lev_rows, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, p->stream, p->A[mode], m, n, Gi,
+                       need, probs, p->chk);
+    ITCPD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_lev_scores<SNT>, dim3(1), dim3(SNT), 0, p->stream, p->A[mode], m, n, W, tau, vwork,
-                       p->smp_iws, probs, p->chk);
+                       p->smp_iws, probs, need, p->chk);
     ITCPD_HIP(hipGetLastError());
     return ITCPD_OK;
 }

@@ -29,6 +29,28 @@ def test_leverage_scores_match_oracle(itcpd, dims, R):
             assert np.abs(got - ref).max() < 1e-13, n
 
 
+def test_leverage_scores_rank_deficient_factor(itcpd):
+    """A factor with two equal columns: the Gram form cannot factor it and the
+    Householder form (the reference's qr) runs.  Its 4th column of Q is then built from
+    rounding noise (numpy's LAPACK lands on another one), so the check is what holds
+    for any such Q: the 4 orthonormal columns contain range(A), hence
+    p(i) >= hat_3(i) / 4 and sum p = 1 (a Gram form run on the tiny pivot fails both)."""
+    rng = np.random.default_rng(5)
+    T = rng.standard_normal((20, 9, 8))
+    init = O.random_CPD(T, 4, np.random.default_rng(6))
+    A0 = init.factors[0].copy()
+    A0[:, 3] = A0[:, 1]
+    fac = [A0] + [f.copy() for f in init.factors[1:]]
+    with itcpd.ALSPlan(T, 4) as plan:
+        plan.set_cpd(itcpd.CPD(fac, init.lam))
+        got = plan.leverage_scores(0)
+    U, sv, _ = np.linalg.svd(A0, full_matrices=False)
+    hat3 = (U[:, :3] ** 2).sum(axis=1)
+    assert sv[3] < 1e-12 * sv[0]
+    assert abs(got.sum() - 1.0) < 1e-12
+    assert (got >= hat3 / 4 - 1e-12).all()
+
+
 STEP_CASES = [((12, 13, 3), 5, 100, np.float64), ((9, 8, 7), 6, 3, np.float64),
               ((20, 18, 16, 5), 80, 400, np.float64), ((12, 13, 3), 5, 60, np.float32)]
 
