@@ -1079,8 +1079,11 @@ namespace {
 
 // rows per wave in the row solves (fewer for large R to bound registers)
 template <int RQ>
-constexpr int rpw_of() { return RQ <= 4 ? 4 : (RQ == 8 ? 2 : 1); }
-int rpb_of(int R) { return 4 * (R <= 256 ? 4 : (R <= 512 ? 2 : 1)); }
+// 64 < R <= 128: one row per wave, so a mode of a few hundred rows spreads over
+// as many workgroups (each stages U in LDS; 4 interleaved rows per wave left most
+// CUs idle at I = 224)
+constexpr int rpw_of() { return RQ == 2 ? 1 : (RQ <= 4 ? 4 : (RQ == 8 ? 2 : 1)); }
+int rpb_of(int R) { return 4 * (R <= 64 ? 4 : (R <= 128 ? 1 : (R <= 256 ? 4 : (R <= 512 ? 2 : 1)))); }
 
 // Block count of the row solves, capped so that the per-block Gram partials stay
 // within 2^25 doubles.
