@@ -17,7 +17,7 @@ All arithmetic runs in libitcpd_hip.so on the GPU; there is no CPU fallback.
 """
 from .api import (  # noqa: F401
     CPD, ALSPlan, BatchALS, DeviceTensor, MLP, reconstruct, CPDFit, FitCheck, CPDiffCheck, CPAngleCheck, KRPFreeNormal, KRPNormal, NoCheck, CPDOptimizer,
-    LevScoreSampled, ProjectionAlgorithm,
+    LevScoreSampled, BlockLevScoreSampled, ProjectionAlgorithm,
     als_optimize, cp_rank, decompose, decompose_adaptive, decompose_batch, increase_cpd_rank,
     random_CPD, tensor_norm,
 )
@@ -25,7 +25,7 @@ from ._lib import ITCPDError  # noqa: F401
 
 __all__ = [
     "CPD", "ALSPlan", "BatchALS", "decompose_batch", "DeviceTensor", "MLP", "reconstruct", "CPDFit", "FitCheck", "CPDiffCheck", "CPAngleCheck", "KRPFreeNormal", "KRPNormal", "NoCheck",
-    "LevScoreSampled", "ProjectionAlgorithm",
+    "LevScoreSampled", "BlockLevScoreSampled", "ProjectionAlgorithm",
     "CPDOptimizer", "als_optimize", "cp_rank", "decompose", "decompose_adaptive",
     "increase_cpd_rank", "random_CPD", "tensor_norm", "ITCPDError",
 ]

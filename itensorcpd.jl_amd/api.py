@@ -46,6 +46,26 @@ class LevScoreSampled(ProjectionAlgorithm):
     def nsamples(self, mode: int) -> int:
         return self.NSamples[0] if len(self.NSamples) == 1 else self.NSamples[mode]
 
+    def draw(self, mode: int, weights, rng) -> np.ndarray:
+        """project_krp's draw (krp_lev_score_sampled.jl:27): sample_factor_matrices."""
+        return _sample_factor_matrices(self.nsamples(mode), mode, weights, rng)
+
+
+class BlockLevScoreSampled(LevScoreSampled):
+    """krp_lev_score_sampled.jl:62-103: the first factor other than the solved one is
+    drawn in blocks of consecutive indices (Blocks = one block size for every mode or
+    one per mode), the other factors once per block."""
+
+    def __init__(self, nsamples=0, blocks=1):
+        super().__init__(nsamples)
+        self.Blocks = (int(blocks),) if np.isscalar(blocks) else tuple(int(b) for b in blocks)
+
+    def blocks(self, mode: int) -> int:
+        return self.Blocks[0] if len(self.Blocks) == 1 else self.Blocks[mode]
+
+    def draw(self, mode: int, weights, rng) -> np.ndarray:
+        return _block_sample_factor_matrices(self.nsamples(mode), weights, self.blocks(mode), mode, rng)
+
 
 def _alg_code(alg) -> int:
     if alg is None or isinstance(alg, KRPFreeNormal) or alg is KRPFreeNormal:
@@ -499,12 +519,12 @@ class ALSPlan:
         rng = np.random.default_rng(rng)
         N = len(self.dims)
         weights = [self.leverage_scores(m) for m in range(N)]
-        samples = [_sample_factor_matrices(alg.nsamples(m), m, weights, rng) for m in range(N)]
+        samples = [alg.draw(m, weights, rng) for m in range(N)]
         it, sweeps = check.iter, 0
         while it < check.max_counter:
             for n in range(N):
                 if stop_resample < 0 or stop_resample > check.iter:
-                    samples[n] = _sample_factor_matrices(alg.nsamples(n), n, weights, rng)
+                    samples[n] = alg.draw(n, weights, rng)
                 self.sampled_solve(n, samples[n], normal)
                 weights[n] = self.leverage_scores(n)
             sweeps += 1
@@ -615,6 +635,44 @@ def _sample_factor_matrices(nsamps: int, skip: int, probs, rng) -> np.ndarray:
         w = np.abs(p)
         cols.append(rng.choice(len(w), size=nsamps, replace=True, p=w / w.sum()).astype(np.int32))
     return np.stack(cols, axis=1)
+
+
+def _draw_one(p, rng) -> int:
+    w = np.abs(p)
+    return int(rng.choice(len(w), size=1, replace=True, p=w / w.sum())[0])
+
+
+def _block_sample_factor_matrices(nsamps: int, probs, block_size: int, skip: int, rng) -> np.ndarray:
+    """probability.jl:66-107: the first factor other than `skip` is cut into
+    size // block_size blocks (the first size % block_size one index longer) weighted by
+    their summed probabilities; each of nsamps // block_size rounds draws one index of
+    every other factor (sample_single_col_from_factors, :16-18), then a block, and fills
+    consecutive rows with the block's indices; rows left over take single draws.  0-based."""
+    blocked = 1 if skip == 0 else 0
+    bf = np.asarray(probs[blocked])
+    nblocks, resid = len(bf) // block_size, len(bf) % block_size
+    if nblocks == 0:
+        raise ValueError("block size larger than the blocked factor")
+    starts, bprob, m = [0], np.empty(nblocks), 0
+    for i in range(nblocks):
+        ln = block_size + (1 if i < resid else 0)
+        bprob[i] = bf[m:m + ln].sum()
+        m += ln
+        starts.append(m)
+    out = np.empty((nsamps, len(probs) - 1), dtype=np.int32)
+    row = 0
+    for _ in range(nsamps // block_size):
+        other = [_draw_one(p, rng) for q, p in enumerate(probs) if q != skip]
+        b = int(rng.choice(nblocks, p=bprob / bprob.sum()))
+        for idx in range(starts[b], starts[b + 1]):
+            if row >= nsamps:
+                break
+            out[row, 0] = idx
+            out[row, 1:] = other[1:]
+            row += 1
+    for r in range(row, nsamps):
+        out[r, :] = [_draw_one(p, rng) for q, p in enumerate(probs) if q != skip]
+    return out
 
 
 def _optimize_sampled(target, cp: CPD, alg: "LevScoreSampled", check, normal: bool, stop_resample: int,

@@ -42,6 +42,48 @@ def sample_factor_matrices(nsamps: int, skip: int, probs, rng) -> np.ndarray:
     return np.stack(cols, axis=1)
 
 
+def sample_single_col_from_factors(skip: int, probs, rng) -> list:
+    """probability.jl:16-18: one draw from each factor's weights except ``skip``'s."""
+    return [int(samples_from_probability_vector(p, 1, rng)[0]) for m, p in enumerate(probs) if m != skip]
+
+
+def block_sample_factor_matrices(nsamps: int, probs, block_size: int, skip: int, rng) -> np.ndarray:
+    """probability.jl:66-107 (BlockLevScoreSampled): the first factor other than
+    ``skip`` is cut into size_fast // block_size blocks (the first size_fast % block_size
+    one index longer) weighted by their summed probabilities; each of nsamps //
+    block_size rounds draws one index of every other factor, then a block, and fills
+    consecutive rows with the block's indices (the other modes fixed to the round's
+    draws); rows left over take single draws.  0-based, -> (nsamps, N-1)."""
+    blocked = 1 if skip == 0 else 0
+    bf = np.asarray(probs[blocked])
+    size_fast = len(bf)
+    nblocks, resid = size_fast // block_size, size_fast % block_size
+    if nblocks == 0:
+        raise ValueError("block size larger than the blocked factor")
+    starts = [0]
+    bprob = np.empty(nblocks)
+    m = 0
+    for i in range(nblocks):
+        ln = block_size + (1 if i < resid else 0)
+        bprob[i] = bf[m:m + ln].sum()
+        m += ln
+        starts.append(m)
+    out = np.empty((nsamps, len(probs) - 1), dtype=np.int32)
+    row = 0
+    for _ in range(nsamps // block_size):
+        other = sample_single_col_from_factors(skip, probs, rng)
+        b = int(rng.choice(nblocks, p=bprob / bprob.sum()))
+        for idx in range(starts[b], starts[b + 1]):
+            if row >= nsamps:
+                break
+            out[row, 0] = idx
+            out[row, 1:] = other[1:]
+            row += 1
+    for r in range(row, nsamps):
+        out[r, :] = sample_single_col_from_factors(skip, probs, rng)
+    return out
+
+
 def pivot_hadamard(others, samples: np.ndarray) -> np.ndarray:
     """had_contract.jl:277-295: prod over the other factors of their sampled rows."""
     out = np.ones((samples.shape[0], others[0].shape[1]))
@@ -88,25 +130,37 @@ def _projection_check(check, factors, lam) -> bool:
 
 
 def optimize_sampled(T: np.ndarray, cp: CPD, nsamples, check=None, normal: bool = False,
-                     stop_resample: int = -1, rng=None) -> CPD:
+                     stop_resample: int = -1, rng=None, blocks=None) -> CPD:
     """compute_als(::LevScoreSampled) (optimizers/.../krp_lev_score_sampled.jl:1-37) and
     optimize (optimize.jl:6-32) with project_krp / post_solve of
-    krp_lev_score_sampled.jl:18-57.  nsamples: int or one per mode."""
+    krp_lev_score_sampled.jl:18-57.  nsamples: int or one per mode.  With ``blocks``
+    (int or one per mode) the BlockLevScoreSampled draws (krp_lev_score_sampled.jl:
+    66-103, optimizers/.../krp_lev_score_sampled.jl:39-83)."""
     rng = np.random.default_rng(rng)
     N = T.ndim
     ns = [nsamples] * N if np.isscalar(nsamples) else list(nsamples)
     if len(ns) == 1:
         ns = ns * N
+    bs = None
+    if blocks is not None:
+        bs = [blocks] * N if np.isscalar(blocks) else list(blocks)
+        if len(bs) == 1:
+            bs = bs * N
+
+    def draw(n, weights):
+        if bs is None:
+            return sample_factor_matrices(ns[n], n, weights, rng)
+        return block_sample_factor_matrices(ns[n], weights, bs[n], n, rng)
     check = check if check is not None else NoCheck(100)
     factors = [np.array(A, dtype=np.float64) for A in cp.factors]
     lam = np.array(cp.lam, dtype=np.float64)
     weights = [leverage_score_probability(A) for A in factors]
-    samples = [sample_factor_matrices(ns[m], m, weights, rng) for m in range(N)]
+    samples = [draw(m, weights) for m in range(N)]
     it = check.iter
     while it < check.max_counter:
         for n in range(N):
             if stop_resample < 0 or stop_resample > check.iter:
-                samples[n] = sample_factor_matrices(ns[n], n, weights, rng)
+                samples[n] = draw(n, weights)
             factors[n], lam = sampled_solve(T, factors, n, samples[n], normal)
             weights[n] = leverage_score_probability(factors[n])
         if _projection_check(check, factors, lam):

@@ -134,3 +134,23 @@ def test_out_of_range_sample_is_rejected(itcpd):
             plan.sampled_solve(0, np.array([[0, 3]], dtype=np.int32))   # mode 2 has 3 indices
         with pytest.raises(ValueError):
             plan.sampled_solve(0, np.zeros((4, 3), dtype=np.int32))
+
+
+@pytest.mark.parametrize("blocks", [1, 12])
+def test_block_sampled_als_matches_oracle(itcpd, blocks):
+    """BlockLevScoreSampled((50, 50, 500), blocks) (rand_cp_als.jl:133-149): a whole run
+    with the same draws as the oracle, and the reference's bar (best of 3 < 0.1)."""
+    cpT = O.random_CPD(np.zeros((12, 13, 3)), 5, np.random.default_rng(101))
+    T = O.reconstruct(cpT.factors, cpT.lam)
+    cp0 = O.random_CPD(T, 5, np.random.default_rng(201))
+    errs = []
+    for tr in range(3):
+        got = itcpd.als_optimize(T, itcpd.CPD(cp0.factors, cp0.lam),
+                                 alg=itcpd.BlockLevScoreSampled((50, 50, 500), blocks),
+                                 check=itcpd.CPAngleCheck(1e-5, 10), rng=301 + tr)
+        ref = S.optimize_sampled(T, cp0, (50, 50, 500), check=O.CPAngleCheck(1e-5, 10), rng=301 + tr,
+                                 blocks=blocks)
+        for a, b in zip(got.factors, ref.factors):
+            assert rel(a, b) < 1e-8
+        errs.append(rel(O.reconstruct(got.factors, got.lam), T))
+    assert min(errs) < 0.1

@@ -125,3 +125,59 @@ def test_stop_resample_zero_keeps_the_first_draws():
         for n in range(3):
             f[n], lam = S.sampled_solve(T, f, n, smp[n], True)
     assert max(np.abs(a - b).max() for a, b in zip(out.factors, f)) == 0.0
+
+
+def test_block_sampler_structure():
+    """probability.jl:66-107: full rounds fill runs of consecutive indices of one block
+    of the first non-skipped factor with the other modes fixed; leftovers are single
+    draws; the blocks cover the factor (the first size % block_size one longer)."""
+    rng = np.random.default_rng(11)
+    probs = [np.full(7, 1 / 7), np.arange(1, 11) / 55.0, np.full(4, 0.25)]
+    smp = S.block_sample_factor_matrices(17, probs, 3, 0, rng)   # blocked: mode 1 (10 = 4+3+3)
+    assert smp.shape == (17, 2)
+    starts = [0, 4, 7, 10]
+    row = 0
+    for _ in range(17 // 3):
+        b = [k for k in range(3) if starts[k] == smp[row, 0]]
+        assert len(b) == 1
+        ln = starts[b[0] + 1] - starts[b[0]]
+        run = smp[row:row + ln]
+        assert (run[:, 0] == np.arange(starts[b[0]], starts[b[0]] + len(run))).all()
+        assert (run[:, 1] == run[0, 1]).all()
+        row += len(run)
+        if row >= 17:
+            break
+    assert ((smp[:, 0] >= 0) & (smp[:, 0] < 10)).all() and ((smp[:, 1] >= 0) & (smp[:, 1] < 4)).all()
+    with pytest.raises(ValueError):
+        S.block_sample_factor_matrices(5, probs, 11, 0, rng)
+
+
+@pytest.mark.parametrize("blocks", [1, 12])
+def test_block_reference_bar(blocks):
+    """rand_cp_als.jl:133-149: BlockLevScoreSampled((50, 50, 500), blocks), best of 3 tries
+    within relative error 0.1 (these seeds meet it)."""
+    cpT = O.random_CPD(np.zeros((12, 13, 3)), 5, np.random.default_rng(101))
+    T = O.reconstruct(cpT.factors, cpT.lam)
+    cp0 = O.random_CPD(T, 5, np.random.default_rng(201))
+    errs = []
+    for tr in range(3):
+        out = S.optimize_sampled(T, cp0, (50, 50, 500), check=O.CPAngleCheck(1e-5, 10), rng=301 + tr,
+                                 blocks=blocks)
+        errs.append(np.linalg.norm(O.reconstruct(out.factors, out.lam) - T) / np.linalg.norm(T))
+    assert min(errs) < 0.1
+
+
+def test_product_samplers_draw_what_the_oracle_draws():
+    """The engine's host-side samplers (the reference draws on the host too) consume the
+    generator exactly as the oracle's restatements do, so GPU runs and oracle runs with
+    one seed see the same samples."""
+    import __graft_entry__ as G
+    E = G.load_package()
+    probs = [np.random.default_rng(1).random(n) for n in (9, 13, 5)]
+    for skip in range(3):
+        a = E.LevScoreSampled(40).draw(skip, probs, np.random.default_rng(7))
+        b = S.sample_factor_matrices(40, skip, probs, np.random.default_rng(7))
+        assert (a == b).all()
+        a = E.BlockLevScoreSampled(40, 4).draw(skip, probs, np.random.default_rng(8))
+        b = S.block_sample_factor_matrices(40, probs, 4, skip, np.random.default_rng(8))
+        assert (a == b).all()
