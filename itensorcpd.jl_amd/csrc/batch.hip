@@ -344,7 +344,7 @@ __device__ void batch_update(int n, int D, int R, const BatchLds& L) {
 }
 
 template <int NT, int RB>
-__device__ __noinline__ void batch_problem(const BatchProblem& pb, const double* __restrict__ Tall,
+__device__ __forceinline__ void batch_problem(const BatchProblem& pb, const double* __restrict__ Tall,
                               double* __restrict__ factors, double* __restrict__ lambdas,
                               double* __restrict__ fits, int sweeps, const BatchLds& L) {
     const int t = threadIdx.x;
