@@ -101,7 +101,7 @@ def test_float32_tensor(itcpd):
 
 
 def test_fitcheck_trace_and_state(itcpd):
-    # BASELINE config 1 restated: 64^3, rank 10 + N(0, 1e-3), FitCheck(1e-6, 100, norm(T))
+    # the CPU-sized workload: 64^3, rank 10 + N(0, 1e-3), FitCheck(1e-6, 100, norm(T))
     T, _ = lowrank((64, 64, 64), 10, 1e-3, 5)
     init = O.random_CPD(T, 10, np.random.default_rng(6))
     nrm = float(np.linalg.norm(T))
@@ -388,7 +388,7 @@ def test_input_cpd_not_mutated_and_verbose(itcpd, capsys):
 
 # ------------------------------------------------ full-size plain FP64 parity
 def test_plain_fp64_512_cubed_50_sweeps(itcpd):
-    """512^3 FP64, rank-64 N(0,1) factor sum + N(0, 1e-2) noise,
+    """The plain FP64 workload: 512^3 FP64, rank-64 N(0,1) factor sum + N(0, 1e-2) noise,
     rank 64, 50 sweeps from the same init: factors and fit within 1e-10."""
     I, R = 512, 64
     rng = np.random.default_rng(2024)
