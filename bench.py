@@ -38,6 +38,7 @@ ROOT = Path(__file__).resolve().parent
 METRIC = ("CP-ALS sweeps/s + time-to-fit within 1e-6 of FP64: mixed/warm vs plain FP64")
 FP64_MFMA_PEAK_TFLOPS = 78.6
 TRAFFIC_FILE = ROOT / "profiles" / "mttkrp_gemm_512cubed_r64_hbm_traffic.txt"
+LP_TRAFFIC_FILE = ROOT / "profiles" / "lp_ring_gemm_224x4_r128_kernel_stats.md"
 
 
 def synthetic_tensor(I: int, R: int, seed: int) -> np.ndarray:
@@ -66,14 +67,14 @@ def sweep_flops(dims, R) -> float:
     return gemm + ttv + gram + solve
 
 
-def read_traffic():
+def read_traffic(path=TRAFFIC_FILE, keys="left|right"):
     """HBM bytes per launch of the left/right GEMM, measured with rocprofv3 PMC
     counters (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated); None if absent."""
-    if not TRAFFIC_FILE.exists():
+    if not path.exists():
         return None
     out = {}
-    for line in TRAFFIC_FILE.read_text().splitlines():
-        m = re.match(r"\s*traffic_bytes_per_launch_(left|right)\s*[:=]\s*([0-9.eE+]+)", line)
+    for line in path.read_text().splitlines():
+        m = re.match(r"\s*traffic_bytes_per_launch_(" + keys + r")\s*[:=]\s*([0-9.eE+]+)", line)
         if m:
             out[m.group(1)] = float(m.group(2))
     return out or None
@@ -238,7 +239,8 @@ def bench_precision(args, E) -> dict:
         # mixed time-to-fit is FP32 sweeps); the BF16x3 one is HBM-bound on T
         "roofline": {"bound": "mfma", "achieved": kstats["fp32_left"]["tflops"], "peak": 157.3,
                      "unit": "TFLOP/s", "frac": kstats["fp32_left"]["tflops"] / 157.3,
-                     "traffic": None, "kernel": "FP32 left ring GEMM (2*I^4*R flop/launch)",
+                     "traffic": (read_traffic(LP_TRAFFIC_FILE, "fp32_left") or {}).get("fp32_left"),
+                     "kernel": "FP32 left ring GEMM (2*I^4*R flop/launch)",
                      "kernels": kstats,
                      "bf16x3_left_hbm_frac": kstats["bf16x3_left"]["T_GBps"] / 8000.0,
                      "sweep_fp64_tflops": flops / (ms_plain * 1e-3) / 1e12,
