@@ -187,6 +187,38 @@ int itcpd_leverage_scores(itcpd_plan* plan, int32_t mode, double* probs, int32_t
 int itcpd_sampled_solve(itcpd_plan* plan, int32_t mode, int64_t nsamps, const int32_t* samples,
                         int32_t mem, int32_t normal, int32_t* info);
 
+/* Column-pivoted QR of the mode-`mode` unfolding of the plan's tensor: Julia's
+ * qr(Tmat, ColumnNorm()) = LAPACK dgeqp3 in compute_als(::QRPivProjected)
+ * (optimizers/als_optimizers/randomized/qr_lev_score_sampled.jl:23-24) and, for a
+ * column subset, in SEQRCS (SEQRCS.jl:160).  Tmat = reshape(array(T, (i_mode,
+ * others...)), (I_mode, prod others)): column c enumerates the other modes with the
+ * first one fastest.  cols (host, ncols entries, 0-based; NULL: every column, and
+ * ncols must then be prod of the other dims) restricts the QR to those columns in
+ * that order.  jpvt (host, ncols) = the column order as 0-based positions into the
+ * list, rdiag (host, min(I_mode, ncols)) = the diagonal of R.  A set-up step before
+ * the sweeps (one workgroup on the device); synchronous. */
+int itcpd_qr_pivots(itcpd_plan* plan, int32_t mode, const int64_t* cols, int64_t ncols,
+                    int64_t* jpvt, double* rdiag);
+
+/* The same for the sketch T_(mode) Omega^T of SEQRCS (SEQRCS.jl:140-146:
+ * sketched_matricization, pivot_mapping.jl:103-139, then qr!(A_sk, ColumnNorm())).
+ * Omega (l x prod others) is given as s row indices (0-based, < l) and s values per
+ * column (host, column after column, as itcpd_sparse_sign writes them).  jpvt (host,
+ * l), rdiag (host, min(I_mode, l)). */
+int itcpd_sketch_qr_pivots(itcpd_plan* plan, int32_t mode, int64_t l, int32_t s,
+                           const int32_t* rows, const double* vals, int64_t* jpvt, double* rdiag);
+
+/* Sparse sign embedding d x m with zeta nonzeros per column, replacing the
+ * reference's native sparse_sign (sparse_sign.c:25-69: distinct rows per column) and
+ * sparsestack (sparsestack.c:24-78, injective != 0: entry q in block q of the
+ * near-equal split of 0..d-1), bound in SEQRCS.jl:41-60.  Values +-1/sqrt(zeta),
+ * zeta clamped to d.  The reference draws from libc rand(); here every column has
+ * its own splitmix64 stream derived from `seed`, generated on the device.  vals,
+ * rows (0-based): host, zeta per column, column after column; colstarts (host,
+ * nullable, m + 1 entries) = c * zeta. */
+int itcpd_sparse_sign(int32_t d, int64_t m, int32_t zeta, int32_t injective, uint64_t seed,
+                      double* vals, int32_t* rows, int32_t* colstarts, int32_t device);
+
 /* check_converge of a ProjectionAlgorithm with a CPDiffCheck or CPAngleCheck
  * (ProjectionAlgorithm.jl:53 -> cp_diff_check.jl:20-71 / cp_angle_check.jl:20-73):
  * one evaluation of the check on the plan's current CPD, its previous CPD held by

@@ -63,6 +63,10 @@ SIGNATURES = {
     "itcpd_sampled_solve": (C.c_int, [_P, C.c_int32, C.c_int64, _P, C.c_int32, C.c_int32,
                                       C.POINTER(C.c_int32)]),
     "itcpd_check_cpd": (C.c_int, [_P, C.POINTER(CheckStruct)]),
+    "itcpd_qr_pivots": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, _P]),
+    "itcpd_sketch_qr_pivots": (C.c_int, [_P, C.c_int32, C.c_int64, C.c_int32, _P, _P, _P, _P]),
+    "itcpd_sparse_sign": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_uint64, _P, _P, _P,
+                                    C.c_int32]),
     "itcpd_plan_set_precision": (C.c_int, [_P, C.c_int32]),
     "itcpd_plan_set_profiling": (C.c_int, [_P, C.c_int32]),
     "itcpd_get_kernel_stats": (C.c_int, [_P, C.c_int32, C.POINTER(KernelStats)]),

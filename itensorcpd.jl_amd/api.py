@@ -67,6 +67,106 @@ class BlockLevScoreSampled(LevScoreSampled):
         return _block_sample_factor_matrices(self.nsamples(mode), weights, self.blocks(mode), mode, rng)
 
 
+class QRPivProjected(ProjectionAlgorithm):
+    """algorithms/als_algorithms/randomized/qr_lev_score_sampled.jl:9-20: the samples of
+    mode n are the pivots start..end (1-based, inclusive; end 0 = every column) of a
+    column-pivoted QR of the unfolding T_(n), fixed for the whole run.  Start / End: one
+    value for every mode or one per mode."""
+
+    def __init__(self, start=None, end=None):
+        if start is not None and end is None:
+            # QRPivProjected(n) = (1, n); QRPivProjected(n::Tuple) = (ones, n)
+            start, end = ((1,) * len(start), start) if not np.isscalar(start) else (1, start)
+        self.Start = 1 if start is None else start
+        self.End = 0 if end is None else end
+
+    def start(self, mode: int) -> int:
+        return int(self.Start if np.isscalar(self.Start) else
+                   (self.Start[0] if len(self.Start) == 1 else self.Start[mode]))
+
+    def stop(self, mode: int) -> int:
+        return int(self.End if np.isscalar(self.End) else
+                   (self.End[0] if len(self.End) == 1 else self.End[mode]))
+
+    def copy_alg(self, new_start=0, new_end=0):
+        """copy_alg (qr_lev_score_sampled.jl:19-20): a copy with a new range."""
+        out = type(self).__new__(type(self))
+        out.__dict__.update(self.__dict__)
+        out.Start = self.Start if np.all(np.asarray(new_start) == 0) else new_start
+        out.End = self.End if np.all(np.asarray(new_end) == 0) else new_end
+        return out
+
+
+class SEQRCSPivProjected(QRPivProjected):
+    """qr_lev_score_sampled.jl:26-52: as QRPivProjected, but for the modes in
+    random_modes (0-based here) the pivots come from SE-QRCS (SEQRCS.jl:127-175): a
+    column-pivoted QR of the sparse sketch T_(n) Omega^T, its first rank_vect[n]
+    pivots mapped back to the columns of T_(n) they touch, and a column-pivoted QR of
+    those columns.  rank_vect: one value per random mode (or a dict mode -> value)."""
+
+    def __init__(self, start=None, end=None, random_modes=None, rank_vect=None):
+        super().__init__(start, end)
+        self.random_modes = None if random_modes is None else tuple(int(m) for m in random_modes)
+        if rank_vect is not None and not isinstance(rank_vect, dict):
+            rank_vect = dict(zip(self.random_modes, (int(r) for r in rank_vect)))
+        self.rank_vect = rank_vect
+
+
+class PivotSetup:
+    """What compute_als(::QRPivProjected / ::SEQRCSPivProjected) keeps in
+    additional_items (qr_lev_score_sampled.jl:62-70): the per-mode reference pivots
+    (column order of the QR, 0-based columns), the shuffled pivots as multi-indices,
+    the effective ranks and the samples of the current range."""
+
+    def __init__(self, alg, ref_pivs, pivots, effective_ranks, samples, normal, odims):
+        self.alg = alg
+        self.odims = odims
+        self.ref_pivs = ref_pivs
+        self.pivots = pivots
+        self.effective_ranks = effective_ranks
+        self.samples = samples
+        self.normal = normal
+
+
+def _pivot_range(alg, mode: int, ncols: int):
+    """qr_lev_score_sampled.jl:33-41: the 1-based inclusive range start..end of the
+    pivot list that becomes the sample set (end 0 or beyond the columns = every column)."""
+    e = alg.stop(mode)
+    e = ncols if e == 0 else min(e, ncols)
+    b = alg.start(mode)
+    if not (0 < b <= e):
+        raise ValueError("sample range start must lie in 1..end")
+    return b, e
+
+
+def _column_to_multi_coords(cols, dims) -> np.ndarray:
+    """pivot_mapping.jl:17-35: column index -> one 0-based index per other mode, the
+    first one fastest."""
+    cols = np.asarray(cols, dtype=np.int64)
+    out = np.empty((cols.size, len(dims)), dtype=np.int32)
+    rem = cols.copy()
+    for q, d in enumerate(dims):
+        out[:, q] = rem % d
+        rem //= d
+    return out
+
+
+def sparse_sign(d: int, m: int, zeta: int, injective: bool = False, seed: int = 0, device: int = 0):
+    """The sparse sign embedding of SEQRCS (sparse_sign.c:25-69, sparsestack.c:24-78),
+    d x m with min(zeta, d) nonzeros per column, generated on the device.  Returns
+    (vals, rows, colstarts): zeta values and 0-based rows per column, column after
+    column, and the column starts."""
+    z = min(int(zeta), int(d))
+    vals = np.empty(m * z)
+    rows = np.empty(m * z, dtype=np.int32)
+    colstarts = np.empty(m + 1, dtype=np.int32)
+    lib = L.load()
+    L.check(lib.itcpd_sparse_sign(int(d), int(m), int(zeta), 1 if injective else 0, C.c_uint64(seed & (2**64 - 1)),
+                                  C.c_void_p(vals.ctypes.data), C.c_void_p(rows.ctypes.data),
+                                  C.c_void_p(colstarts.ctypes.data), int(device)))
+    return vals, rows, colstarts
+
+
 def _alg_code(alg) -> int:
     if alg is None or isinstance(alg, KRPFreeNormal) or alg is KRPFreeNormal:
         return L.ITCPD_ALG_KRP_FREE_NORMAL
@@ -386,6 +486,7 @@ class ALSPlan:
                                       _alg_code(alg), dev))
         self._h = h
         self._lib = lib
+        self.device = dev
         if self._t.mem == L.ITCPD_MEM_HOST:
             self._t.keep = None          # copied to the device; host copy no longer needed
 
@@ -528,25 +629,137 @@ class ALSPlan:
                 self.sampled_solve(n, samples[n], normal)
                 weights[n] = self.leverage_scores(n)
             sweeps += 1
-            if isinstance(check, FitCheck):
-                # a FitCheck only counts sweeps for these algorithms
-                check.iter += 1
-                if check.iter >= check.max_counter:
-                    check.iter = 0
-                stop = False
-            elif isinstance(check, NoCheck):
-                check.iter += 1
-                stop = check.iter == check.max_counter
-                if stop:
-                    check.iter = 0
-            else:
-                stop = self.check_cpd(check)
+            stop = self._projection_check(check)
             if verbose:
                 print(f"{self.rank}\t {check.iter}")
             if stop:
                 break
             it += 1
         return sweeps
+
+    # -- pivot-based projections (QRPivProjected, SEQRCSPivProjected)
+    def qr_pivots(self, mode: int, cols=None):
+        """Column-pivoted QR (qr(A, ColumnNorm())) of the mode-`mode` unfolding, or of the
+        listed 0-based columns of it.  Returns (column order as 0-based positions into
+        the list, diagonal of R)."""
+        nall = int(np.prod([d for m, d in enumerate(self.dims) if m != mode]))
+        if cols is None:
+            n, cptr = nall, None
+        else:
+            c = np.ascontiguousarray(cols, dtype=np.int64)
+            n, cptr = c.size, C.c_void_p(c.ctypes.data)
+        jpvt = np.empty(n, dtype=np.int64)
+        rdiag = np.empty(min(self.dims[mode], n))
+        L.check(self._lib.itcpd_qr_pivots(self._h, mode, cptr, n, C.c_void_p(jpvt.ctypes.data),
+                                          C.c_void_p(rdiag.ctypes.data)))
+        return jpvt, rdiag
+
+    def sketch_qr_pivots(self, mode: int, l: int, s: int, rows, vals):
+        """Column-pivoted QR of the sketch T_(mode) Omega^T (Omega: l rows, s nonzeros per
+        column given as 0-based rows and values, column after column)."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        nall = int(np.prod([d for m, d in enumerate(self.dims) if m != mode]))
+        if r.size != nall * s or v.size != nall * s:
+            raise ValueError("the embedding needs s entries per column of the unfolding")
+        jpvt = np.empty(l, dtype=np.int64)
+        rdiag = np.empty(min(self.dims[mode], l))
+        L.check(self._lib.itcpd_sketch_qr_pivots(self._h, mode, int(l), int(s), C.c_void_p(r.ctypes.data),
+                                                 C.c_void_p(v.ctypes.data), C.c_void_p(jpvt.ctypes.data),
+                                                 C.c_void_p(rdiag.ctypes.data)))
+        return jpvt, rdiag
+
+    def seqrcs(self, mode: int, l: int, s: int, t: int, rows, vals):
+        """SEQRCS(::Val{false}, ...) (SEQRCS.jl:127-175) with a given embedding: QR-CP of
+        the sketch, its first t pivots mapped back to the columns that touch those sketch
+        rows (pivot by pivot, increasing column, first occurrence kept), QR-CP of that
+        column subset.  Returns (pivots over every column, diagonal of the subset's R)."""
+        nall = int(np.prod([d for m, d in enumerate(self.dims) if m != mode]))
+        p_sk, _ = self.sketch_qr_pivots(mode, l, s, rows, vals)
+        rmat = np.asarray(rows, dtype=np.int64).reshape(nall, s)
+        seen = np.zeros(nall, dtype=bool)
+        indices = []
+        for p in p_sk[:t]:
+            for c in np.nonzero((rmat == p).any(axis=1))[0]:
+                if not seen[c]:
+                    seen[c] = True
+                    indices.append(int(c))
+        indices = np.array(indices, dtype=np.int64)
+        p_sub, rdiag = self.qr_pivots(mode, indices)
+        return np.concatenate([indices[p_sub], np.nonzero(~seen)[0]]), rdiag
+
+    def projection_setup(self, alg: "QRPivProjected", trunc_tol: float = 0.01, shuffle_pivots: bool = True,
+                         normal: bool = True, injective: bool = False, rng=None, seed=None) -> "PivotSetup":
+        """compute_als(::QRPivProjected / ::SEQRCSPivProjected) (optimizers/als_optimizers/
+        randomized/qr_lev_score_sampled.jl:1-174): per mode the pivot order (QR-CP of the
+        unfolding, or SE-QRCS for alg.random_modes), the effective rank (|r_jj| / max >
+        trunc_tol), the tail beyond it shuffled with `rng`, and the samples start..end.
+        The embeddings of SE-QRCS come from `seed` (drawn from rng when None)."""
+        rng = np.random.default_rng(rng)
+        N = len(self.dims)
+        rmodes = getattr(alg, "random_modes", None) or ()
+        rank_sk = getattr(alg, "rank_vect", None)
+        ref_pivs, pivots, meffs, samples = [], [], [], []
+        for n in range(N):
+            odims = [d for m, d in enumerate(self.dims) if m != n]
+            nall = int(np.prod(odims))
+            b, e = _pivot_range(alg, n, nall)
+            m = self.dims[n]
+            if n in rmodes:
+                k_sk = e if rank_sk is None else int(rank_sk[n])
+                l = int(round(3 * m * np.log(m)))
+                s_ = int(round(np.log(m)))
+                sd = int(rng.integers(0, 2**63)) if seed is None else int(seed) + n
+                vals, rows, _ = sparse_sign(l, nall, s_, injective, sd, self.device)
+                p, rdiag = self.seqrcs(n, l, min(s_, l), k_sk, rows, vals)
+            else:
+                p, rdiag = self.qr_pivots(n)
+            ref_pivs.append(p.copy())
+            a = np.abs(rdiag)
+            meff = int((a / a.max() > trunc_tol).sum())
+            tail = p[meff:]
+            if shuffle_pivots:
+                tail = tail[rng.permutation(tail.size)]
+            coords = _column_to_multi_coords(np.concatenate([p[:meff], tail]), odims)
+            pivots.append(coords)
+            meffs.append(meff)
+            samples.append(np.asfortranarray(coords[b - 1:e]))
+        odl = [[d for m, d in enumerate(self.dims) if m != n] for n in range(N)]
+        return PivotSetup(alg, ref_pivs, pivots, meffs, samples, normal, odl)
+
+    def run_projected(self, setup: "PivotSetup", check, verbose: bool = False) -> int:
+        """optimize (optimize.jl:6-32) with the pivot-based project_krp / matricize_tensor
+        (algorithms/.../qr_lev_score_sampled.jl:150-169): the fixed samples of `setup`,
+        no post_solve, the ProjectionAlgorithm check.  Returns the sweeps run."""
+        N = len(self.dims)
+        it, sweeps = check.iter, 0
+        while it < check.max_counter:
+            for n in range(N):
+                self.sampled_solve(n, setup.samples[n], setup.normal)
+            sweeps += 1
+            stop = self._projection_check(check)
+            if verbose:
+                print(f"{self.rank}\t {check.iter}")
+            if stop:
+                break
+            it += 1
+        return sweeps
+
+    def _projection_check(self, check) -> bool:
+        """ProjectionAlgorithm.jl:20-53: a FitCheck only counts sweeps, a NoCheck counts
+        to its maximum, the CP checks run on the device."""
+        if isinstance(check, FitCheck):
+            check.iter += 1
+            if check.iter >= check.max_counter:
+                check.iter = 0
+            return False
+        if isinstance(check, NoCheck):
+            check.iter += 1
+            stop = check.iter == check.max_counter
+            if stop:
+                check.iter = 0
+            return stop
+        return self.check_cpd(check)
 
     def check_cpd(self, check) -> bool:
         """One evaluation of a CPDiffCheck / CPAngleCheck on the current CPD (the check's
@@ -684,15 +897,64 @@ def _optimize_sampled(target, cp: CPD, alg: "LevScoreSampled", check, normal: bo
         return plan.get_cpd()
 
 
+def update_samples(setup: PivotSetup, new_end, reshuffle: bool = False, new_start=0, rng=None) -> PivotSetup:
+    """update_samples (algorithms/.../qr_lev_score_sampled.jl:91-145): the same QR pivots
+    with a new sample range (copy_alg); with reshuffle the tail beyond each effective rank
+    of the reference pivots is shuffled again."""
+    rng = np.random.default_rng(rng)
+    alg = setup.alg.copy_alg(new_start, new_end)
+    pivots = [p.copy() for p in setup.pivots]
+    samples = []
+    for n, (p, meff) in enumerate(zip(setup.ref_pivs, setup.effective_ranks)):
+        if reshuffle:
+            tail = p[meff:]
+            pivots[n] = _column_to_multi_coords(np.concatenate([p[:meff], tail[rng.permutation(tail.size)]]),
+                                                setup.odims[n])
+        b, e = _pivot_range(alg, n, pivots[n].shape[0])
+        samples.append(np.asfortranarray(pivots[n][b - 1:e]))
+    return PivotSetup(alg, setup.ref_pivs, pivots, setup.effective_ranks, samples, setup.normal, setup.odims)
+
+
+def compute_projection(target, cp: CPD, alg: QRPivProjected, trunc_tol: float = 0.01,
+                       shuffle_pivots: bool = True, normal: bool = True, injective: bool = False,
+                       rng=None, seed=None) -> PivotSetup:
+    """compute_als for a pivot-based projection on a fresh plan (see
+    ALSPlan.projection_setup)."""
+    with ALSPlan(target, cp.rank) as plan:
+        return plan.projection_setup(alg, trunc_tol=trunc_tol, shuffle_pivots=shuffle_pivots, normal=normal,
+                                     injective=injective, rng=rng, seed=seed)
+
+
+def optimize_projected(target, cp: CPD, setup: PivotSetup, check, verbose: bool = False) -> CPD:
+    """optimize(cp, als) (optimize.jl:6-32) for a pivot-based projection set up by
+    compute_projection / update_samples."""
+    with ALSPlan(target, cp.rank) as plan:
+        plan.set_cpd(cp)
+        plan.run_projected(setup, check, verbose=verbose)
+        return plan.get_cpd()
+
+
 def als_optimize(target, cp: CPD, alg=None, check=None, maxiter=None, verbose=False, *,
-                 normal: bool = False, stop_resample: int = -1, rng=None) -> CPD:
+                 normal=None, stop_resample: int = -1, rng=None, shuffle_pivots: bool = True,
+                 trunc_tol: float = 0.01, injective: bool = False, seed=None) -> CPD:
     """als_optimizer.jl:15-25 with the defaults of compute_als (:37-55):
     alg = KRPFreeNormal(), check = NoCheck(maxiter or 100).  Returns a new CPD.
-    With a ProjectionAlgorithm (LevScoreSampled) the sampled ALS runs instead; normal,
-    stop_resample are its compute_als keywords and rng seeds the draws."""
+    With a ProjectionAlgorithm the projected ALS runs instead: LevScoreSampled draws
+    (normal defaults to False, stop_resample, rng seeds the draws); QRPivProjected /
+    SEQRCSPivProjected fix the samples from column-pivoted QRs (normal defaults to
+    True; shuffle_pivots, trunc_tol, injective as compute_als; rng seeds the shuffles
+    and, unless `seed` is given, the embeddings)."""
     check = NoCheck(100 if maxiter is None else maxiter) if check is None else check
+    if isinstance(alg, QRPivProjected):
+        with ALSPlan(target, cp.rank) as plan:
+            plan.set_cpd(cp)
+            st = plan.projection_setup(alg, trunc_tol=trunc_tol, shuffle_pivots=shuffle_pivots,
+                                       normal=True if normal is None else bool(normal), injective=injective,
+                                       rng=rng, seed=seed)
+            plan.run_projected(st, check, verbose=verbose)
+            return plan.get_cpd()
     if isinstance(alg, ProjectionAlgorithm):
-        return _optimize_sampled(target, cp, alg, check, normal, stop_resample, rng, verbose)
+        return _optimize_sampled(target, cp, alg, check, bool(normal), stop_resample, rng, verbose)
     with ALSPlan(target, cp.rank, alg=alg) as plan:
         plan.set_cpd(cp)
         plan.run(check, verbose=verbose)

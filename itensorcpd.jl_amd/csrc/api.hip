@@ -697,6 +697,141 @@ int itcpd_sampled_solve(itcpd_plan* p, int32_t mode, int64_t nsamps, const int32
     return ITCPD_OK;
 }
 
+// Column-pivoted QR of the unfolding (or a column subset) or of its sparse sketch;
+// the results come back to the host.
+static int qr_pivots_common(itcpd_plan* p, int32_t mode, const std::vector<int64_t>* cols, int64_t n,
+                            const std::vector<int64_t>* rowptr, const std::vector<int64_t>* colidx,
+                            const std::vector<double>* ovals, int64_t* jpvt, double* rdiag) {
+    const int64_t In = p->dims[mode];
+    int rc;
+    int64_t* d_cols = nullptr;
+    int64_t* d_rowptr = nullptr;
+    int64_t* d_colidx = nullptr;
+    double* d_vals = nullptr;
+    int* d_jpvt = nullptr;
+    double* d_rdiag = nullptr;
+    auto cleanup = [&]() {
+        dfree(d_cols);
+        dfree(d_rowptr);
+        dfree(d_colidx);
+        dfree(d_vals);
+        dfree(d_jpvt);
+        dfree(d_rdiag);
+    };
+    const int64_t kq = std::min(In, n);
+    rc = dmalloc(&d_jpvt, (size_t)n);
+    if (!rc) rc = dmalloc(&d_rdiag, (size_t)kq);
+    if (!rc && cols) {
+        rc = dmalloc(&d_cols, cols->size());
+        if (!rc) rc = copy_in(d_cols, cols->data(), cols->size() * 8, ITCPD_MEM_HOST, p->stream);
+    }
+    if (!rc && rowptr) {
+        rc = dmalloc(&d_rowptr, rowptr->size());
+        if (!rc) rc = dmalloc(&d_colidx, std::max<size_t>(1, colidx->size()));
+        if (!rc) rc = dmalloc(&d_vals, std::max<size_t>(1, ovals->size()));
+        if (!rc) rc = copy_in(d_rowptr, rowptr->data(), rowptr->size() * 8, ITCPD_MEM_HOST, p->stream);
+        if (!rc && !colidx->empty())
+            rc = copy_in(d_colidx, colidx->data(), colidx->size() * 8, ITCPD_MEM_HOST, p->stream);
+        if (!rc && !ovals->empty()) rc = copy_in(d_vals, ovals->data(), ovals->size() * 8, ITCPD_MEM_HOST, p->stream);
+    }
+    if (!rc)
+        rc = launch_qr_pivots(p, mode, d_cols, cols ? (int64_t)cols->size() : n, d_rowptr, d_colidx, d_vals,
+                              rowptr ? n : 0, d_jpvt, d_rdiag);
+    std::vector<int32_t> hj((size_t)n);
+    if (!rc) {
+        const hipError_t e1 = hipMemcpy(hj.data(), d_jpvt, (size_t)n * 4, hipMemcpyDeviceToHost);
+        const hipError_t e2 = hipMemcpy(rdiag, d_rdiag, (size_t)kq * 8, hipMemcpyDeviceToHost);
+        if (e1 != hipSuccess) rc = hip_fail(e1, "hipMemcpy(jpvt)");
+        else if (e2 != hipSuccess) rc = hip_fail(e2, "hipMemcpy(rdiag)");
+    }
+    cleanup();
+    if (rc) return rc;
+    for (int64_t q = 0; q < n; ++q) jpvt[q] = hj[(size_t)q];
+    return ITCPD_OK;
+}
+
+static int64_t other_columns(const itcpd_plan* p, int mode) {
+    int64_t c = 1;
+    for (int m = 0; m < p->N; ++m)
+        if (m != mode) c *= p->dims[m];
+    return c;
+}
+
+int itcpd_qr_pivots(itcpd_plan* p, int32_t mode, const int64_t* cols, int64_t ncols, int64_t* jpvt,
+                    double* rdiag) {
+    if (!p || !jpvt || !rdiag) return fail(ITCPD_E_ARG, "null argument");
+    if (mode < 0 || mode >= p->N) return fail(ITCPD_E_ARG, "mode out of range");
+    const int64_t nall = other_columns(p, mode);
+    if (ncols < 1 || ncols > (int64_t)INT32_MAX) return fail(ITCPD_E_ARG, "ncols out of range");
+    if (!cols && ncols != nall) return fail(ITCPD_E_ARG, "without a column list ncols must be every column");
+    if ((p->dims[mode] > (int64_t)INT32_MAX)) return fail(ITCPD_E_UNSUPPORTED, "mode too long");
+    DeviceGuard dg(p->device);
+    std::vector<int64_t> hc;
+    if (cols) {
+        hc.assign(cols, cols + ncols);
+        for (int64_t c : hc)
+            if (c < 0 || c >= nall) return fail(ITCPD_E_ARG, "column index out of range");
+    }
+    return qr_pivots_common(p, mode, cols ? &hc : nullptr, ncols, nullptr, nullptr, nullptr, jpvt, rdiag);
+}
+
+int itcpd_sketch_qr_pivots(itcpd_plan* p, int32_t mode, int64_t l, int32_t s, const int32_t* rows,
+                           const double* vals, int64_t* jpvt, double* rdiag) {
+    if (!p || !rows || !vals || !jpvt || !rdiag) return fail(ITCPD_E_ARG, "null argument");
+    if (mode < 0 || mode >= p->N) return fail(ITCPD_E_ARG, "mode out of range");
+    if (l < 1 || l > (int64_t)INT32_MAX || s < 1) return fail(ITCPD_E_ARG, "embedding size out of range");
+    if ((p->dims[mode] > (int64_t)INT32_MAX)) return fail(ITCPD_E_UNSUPPORTED, "mode too long");
+    const int64_t nall = other_columns(p, mode);
+    const int64_t nnz = nall * s;
+    // Omega by rows: for row j the columns with a nonzero there, increasing
+    std::vector<int64_t> rowptr((size_t)l + 1, 0);
+    for (int64_t e = 0; e < nnz; ++e) {
+        const int32_t r = rows[e];
+        if (r < 0 || r >= l) return fail(ITCPD_E_ARG, "embedding row out of range");
+        ++rowptr[(size_t)r + 1];
+    }
+    for (int64_t j = 0; j < l; ++j) rowptr[(size_t)j + 1] += rowptr[(size_t)j];
+    std::vector<int64_t> fillp(rowptr.begin(), rowptr.end() - 1);
+    std::vector<int64_t> colidx((size_t)nnz);
+    std::vector<double> ov((size_t)nnz);
+    for (int64_t c = 0; c < nall; ++c)
+        for (int q = 0; q < s; ++q) {
+            const int64_t e = c * s + q;
+            const int64_t at = fillp[(size_t)rows[e]]++;
+            colidx[(size_t)at] = c;
+            ov[(size_t)at] = vals[e];
+        }
+    DeviceGuard dg(p->device);
+    return qr_pivots_common(p, mode, nullptr, l, &rowptr, &colidx, &ov, jpvt, rdiag);
+}
+
+int itcpd_sparse_sign(int32_t d, int64_t m, int32_t zeta, int32_t injective, uint64_t seed, double* vals,
+                      int32_t* rows, int32_t* colstarts, int32_t device) {
+    if (!vals || !rows) return fail(ITCPD_E_ARG, "null argument");
+    if (d < 1 || m < 1 || zeta < 1) return fail(ITCPD_E_ARG, "sizes must be positive");
+    if (zeta > d) zeta = d;
+    if (m > ((int64_t)1 << 40) / zeta) return fail(ITCPD_E_ARG, "embedding too large");
+    DeviceGuard dg(device);
+    const size_t nnz = (size_t)m * (size_t)zeta;
+    double* dv = nullptr;
+    int* dr = nullptr;
+    int rc = dmalloc(&dv, nnz);
+    if (!rc) rc = dmalloc(&dr, nnz);
+    if (!rc) rc = launch_sparse_sign(d, m, zeta, injective != 0, seed, dv, dr, nullptr);
+    if (!rc) {
+        const hipError_t e1 = hipMemcpy(vals, dv, nnz * 8, hipMemcpyDeviceToHost);
+        const hipError_t e2 = hipMemcpy(rows, dr, nnz * 4, hipMemcpyDeviceToHost);
+        if (e1 != hipSuccess) rc = hip_fail(e1, "hipMemcpy(vals)");
+        else if (e2 != hipSuccess) rc = hip_fail(e2, "hipMemcpy(rows)");
+    }
+    dfree(dv);
+    dfree(dr);
+    if (rc) return rc;
+    if (colstarts)
+        for (int64_t c = 0; c <= m; ++c) colstarts[c] = (int32_t)(c * zeta);
+    return ITCPD_OK;
+}
+
 int itcpd_check_cpd(itcpd_plan* p, itcpd_check* c) {
     if (!p || !c) return fail(ITCPD_E_ARG, "null argument");
     if (c->kind != ITCPD_CHECK_CPDIFF && c->kind != ITCPD_CHECK_CPANGLE)

@@ -462,6 +462,127 @@ __global__ __launch_bounds__(256) void k_fill_value(double* __restrict__ out, in
     if (i < m) out[i] = v;
 }
 
+// ------------------------------------------------------------ pivot projections
+// QRPivProjected / SEQRCSPivProjected (qr_lev_score_sampled.jl): the pivots come
+// from column-pivoted QRs of the mode-n unfolding T_(n) (or of a column subset of it,
+// or of its sparse sketch T_(n) Omega^T), computed once before the sweeps.  Column c
+// of T_(n) enumerates the other modes with the first one fastest (Julia's reshape of
+// array(T, (i, others...))).
+
+// W(i, q) = T_(n)(i, cols[q]) (cols null: column q), I_n x ncols column-major.
+template <typename TT>
+__global__ __launch_bounds__(256) void k_unfold_cols(const TT* __restrict__ T, OtherModes om,
+                                                     int64_t mstride, int64_t In,
+                                                     const int64_t* __restrict__ cols, int64_t ncols,
+                                                     double* __restrict__ W) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= In * ncols) return;
+    const int64_t q = idx / In, i = idx - q * In;
+    int64_t c = cols ? cols[q] : q;
+    int64_t off = i * mstride;
+    for (int k = 0; k < om.n; ++k) {
+        off += (c % om.dim[k]) * om.tstride[k];
+        c /= om.dim[k];
+    }
+    W[idx] = (double)T[off];
+}
+
+// A_sk(i, j) = sum over the nonzeros e of row j of Omega (increasing column) of
+// vals[e] T_(n)(i, colidx[e])   (sketched_matricization, pivot_mapping.jl:103-139),
+// I_n x l column-major.  rowptr / colidx / vals: Omega by rows.
+template <typename TT>
+__global__ __launch_bounds__(256) void k_sketch(const TT* __restrict__ T, OtherModes om, int64_t mstride,
+                                                int64_t In, const int64_t* __restrict__ rowptr,
+                                                const int64_t* __restrict__ colidx,
+                                                const double* __restrict__ vals, int64_t l,
+                                                double* __restrict__ W) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= In * l) return;
+    const int64_t j = idx / In, i = idx - j * In;
+    double acc = 0.0;
+    for (int64_t e = rowptr[j]; e < rowptr[j + 1]; ++e) {
+        int64_t c = colidx[e];
+        int64_t off = i * mstride;
+        for (int k = 0; k < om.n; ++k) {
+            off += (c % om.dim[k]) * om.tstride[k];
+            c /= om.dim[k];
+        }
+        acc += vals[e] * (double)T[off];
+    }
+    W[idx] = acc;
+}
+
+// Column-pivoted QR of W (m x n, ld m) in one workgroup (xGEQP3 / xLAQP2 pivots);
+// jpvt (n) = the column order, rdiag (min(m, n)) = the diagonal of R.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_qrcp(double* __restrict__ W, int m, int n, double* __restrict__ tau,
+                                             double* __restrict__ vwork, int* __restrict__ jpvt,
+                                             double* __restrict__ rdiag) {
+    __shared__ double redv[NT];
+    __shared__ int redi[NT];
+    geqp3<NT>(W, m, n, m, true, tau, jpvt, vwork, vwork + n, redv, redi);
+    const int kq = m < n ? m : n;
+    for (int j = threadIdx.x; j < kq; j += NT) rdiag[j] = W[j + (int64_t)m * j];
+}
+
+// splitmix64 (Steele, Lea, Flood 2014): the stream of the sparse sign generator.
+__device__ __forceinline__ uint64_t splitmix_next(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Uniform in [0, n): the 2^64 mod n lowest draws are rejected (at most 2^-33 of them
+// for n < 2^31), so r % n is unbiased; the retry count is bounded.
+__device__ __forceinline__ int64_t uniform_below(uint64_t& s, int64_t n) {
+    const uint64_t un = (uint64_t)n;
+    const uint64_t rej = (0ull - un) % un;
+    uint64_t r = splitmix_next(s);
+    for (int a = 0; a < 64 && r < rej; ++a) r = splitmix_next(s);
+    return (int64_t)(r % un);
+}
+
+// Sparse sign embedding, d x m with zeta nonzeros per column (sparse_sign.c:25-69:
+// distinct rows by rejection; sparsestack.c:24-78 (injective): entry q in block q of
+// the near-equal split of 0..d-1), values +-1/sqrt(zeta), one thread per column, each
+// column its own stream of the seed.  rows / vals: zeta per column, column-major.
+__global__ __launch_bounds__(256) void k_sparse_sign(int d, int64_t m, int zeta, int injective, uint64_t seed,
+                                                     double* __restrict__ vals, int* __restrict__ rows) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= m) return;
+    uint64_t st = seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(c + 1));
+    const double a = 1.0 / sqrt((double)zeta);
+    int* rc = rows + c * zeta;
+    double* vc = vals + c * zeta;
+    const int qb = d / zeta, rb = d % zeta;
+    for (int q = 0; q < zeta; ++q) {
+        int row;
+        if (injective) {
+            const int size = q < rb ? qb + 1 : qb;
+            const int start = q < rb ? q * (qb + 1) : rb * (qb + 1) + (q - rb) * qb;
+            row = start + (int)uniform_below(st, size);
+        } else {
+            // redraw on a repeat; after 4096 repeats (never expected: zeta <= d) the
+            // smallest unused row is taken, so the loop always ends
+            row = -1;
+            for (int a2 = 0; a2 < 4096 && row < 0; ++a2) {
+                const int cand = (int)uniform_below(st, d);
+                bool dup = false;
+                for (int k = 0; k < q; ++k) dup = dup || rc[k] == cand;
+                if (!dup) row = cand;
+            }
+            for (int cand = 0; row < 0 && cand < d; ++cand) {
+                bool dup = false;
+                for (int k = 0; k < q; ++k) dup = dup || rc[k] == cand;
+                if (!dup) row = cand;
+            }
+        }
+        rc[q] = row;
+        vc[q] = (splitmix_next(st) & 1ull) ? a : -a;
+    }
+}
+
 OtherModes other_modes(const itcpd_plan* p, int mode) {
     OtherModes om{};
     int64_t stride = 1;
@@ -602,6 +723,62 @@ int sampled_ws_prepare(itcpd_plan* p, int mode, int64_t ns) {
     const size_t doubles = (size_t)(2 * ns * R + ns * In + R * R + 4 * R);
     const size_t ints = (size_t)(ns * (p->N - 1) + R + 4);
     return ensure_sampled_ws(p, doubles, ints);
+}
+
+// Column-pivoted QR of T_(mode) restricted to the columns `cols` (device, ncols of
+// them; null: every column) or, with `omega_rows` given, of the sketch T_(mode)
+// Omega^T with Omega (l x ncols_all) by rows (omega_rowptr / omega_cols / omega_vals,
+// device).  Scratch is allocated here and freed after (a set-up step before the
+// sweeps); jpvt_dev (n ints) and rdiag_dev (min(I_n, n)) receive the result, with n
+// = ncols or l.  The call synchronises the plan stream.
+int launch_qr_pivots(itcpd_plan* p, int mode, const int64_t* cols, int64_t ncols, const int64_t* omega_rowptr,
+                     const int64_t* omega_cols, const double* omega_vals, int64_t l, int* jpvt_dev,
+                     double* rdiag_dev) {
+    const int64_t In = p->dims[mode];
+    const int64_t n = omega_rowptr ? l : ncols;
+    const OtherModes om = other_modes(p, mode);
+    const int64_t ms = tensor_stride(p, mode);
+    double* W = nullptr;
+    int rc;
+    // W (I_n x n) | tau (min) | vn1, vn2 (2 n)
+    if ((rc = dmalloc(&W, (size_t)(In * n + std::min(In, n) + 2 * n)))) return rc;
+    double* tau = W + In * n;
+    double* vwork = tau + std::min(In, n);
+    const unsigned grid = (unsigned)cdiv(In * n, 256);
+    if (omega_rowptr) {
+        if (p->elt_bytes == 8)
+            hipLaunchKernelGGL(k_sketch<double>, dim3(grid), dim3(256), 0, p->stream, (const double*)p->T, om, ms,
+                               In, omega_rowptr, omega_cols, omega_vals, l, W);
+        else
+            hipLaunchKernelGGL(k_sketch<float>, dim3(grid), dim3(256), 0, p->stream, (const float*)p->T, om, ms,
+                               In, omega_rowptr, omega_cols, omega_vals, l, W);
+    } else {
+        if (p->elt_bytes == 8)
+            hipLaunchKernelGGL(k_unfold_cols<double>, dim3(grid), dim3(256), 0, p->stream, (const double*)p->T, om,
+                               ms, In, cols, ncols, W);
+        else
+            hipLaunchKernelGGL(k_unfold_cols<float>, dim3(grid), dim3(256), 0, p->stream, (const float*)p->T, om,
+                               ms, In, cols, ncols, W);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_qrcp<SNT>, dim3(1), dim3(SNT), 0, p->stream, W, (int)In, (int)n, tau, vwork, jpvt_dev,
+                           rdiag_dev);
+        e = hipGetLastError();
+    }
+    const hipError_t e2 = hipStreamSynchronize(p->stream);
+    dfree(W);
+    if (e != hipSuccess) return hip_fail(e, "qr pivots launch");
+    if (e2 != hipSuccess) return hip_fail(e2, "qr pivots");
+    return ITCPD_OK;
+}
+
+int launch_sparse_sign(int d, int64_t m, int zeta, bool injective, uint64_t seed, double* vals_dev, int* rows_dev,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_sparse_sign, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, st, d, m, zeta,
+                       injective ? 1 : 0, seed, vals_dev, rows_dev);
+    ITCPD_HIP(hipGetLastError());
+    return ITCPD_OK;
 }
 
 }  // namespace itcpd

@@ -1,0 +1,188 @@
+"""Pivot-based projected ALS (QRPivProjected, SEQRCSPivProjected) on the GPU against the
+oracle (oracle/pivoted.py): the column-pivoted QR of unfoldings, column subsets and
+sketches (the same pivots as LAPACK dgeqp3's xLAQP2 steps), the device sparse sign
+embedding's invariants, SE-QRCS's map-back, whole runs with the same samples, and the
+reference's bars (test/rand_cp_als.jl:20-70, 73-83)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import oracle.pivoted as P
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("dims,dt", [((20, 30, 40), np.float64), ((6, 5, 4, 3), np.float64),
+                                     ((9, 7, 11), np.float32), ((64, 3, 5), np.float64)])
+def test_qr_pivots_match_oracle(itcpd, dims, dt):
+    T = np.random.default_rng(1).standard_normal(dims).astype(dt)
+    T64 = T.astype(np.float64)
+    with itcpd.ALSPlan(T, 2) as plan:
+        for n in range(len(dims)):
+            p, r = plan.qr_pivots(n)
+            p0, r0 = P.qrcp(P.unfold(T64, n))
+            assert (p == p0).all(), n
+            assert np.abs(r - r0).max() <= 1e-12 * np.abs(r0).max(), n
+
+
+def test_qr_pivots_of_a_column_subset(itcpd):
+    rng = np.random.default_rng(2)
+    T = rng.standard_normal((15, 12, 10))
+    with itcpd.ALSPlan(T, 2) as plan:
+        for n in range(3):
+            ncol = T.size // T.shape[n]
+            cols = rng.permutation(ncol)[:37]
+            p, r = plan.qr_pivots(n, cols)
+            p0, r0 = P.qrcp(P.unfold(T, n)[:, cols])
+            assert (p == p0).all() and np.abs(r - r0).max() <= 1e-12 * np.abs(r0).max()
+        with pytest.raises(itcpd.ITCPDError):
+            plan.qr_pivots(0, np.array([0, 10_000]))
+
+
+def test_qr_pivots_rank_deficient_unfolding(itcpd):
+    """An exact rank-3 tensor: the first three pivots match, |r_jj| collapses after them
+    and the effective rank is 3."""
+    T = O.reconstruct(O.random_CPD(np.zeros((12, 9, 8)), 3, np.random.default_rng(3)).factors, np.ones(3))
+    with itcpd.ALSPlan(T, 2) as plan:
+        for n in range(3):
+            p, r = plan.qr_pivots(n)
+            p0, r0 = P.qrcp(P.unfold(T, n))
+            assert (p[:3] == p0[:3]).all()
+            assert P.effective_rank(r, 1e-8) == 3
+
+
+@pytest.mark.parametrize("injective", [False, True])
+def test_sparse_sign_invariants(itcpd, injective):
+    d, m, z = 180, 1200, 3
+    vals, rows, cs = itcpd.sparse_sign(d, m, z, injective=injective, seed=7)
+    assert P.sparse_sign_structure_ok(d, m, z, vals, rows, injective)
+    assert (cs == np.arange(m + 1) * z).all()
+    v2, r2, _ = itcpd.sparse_sign(d, m, z, injective=injective, seed=7)
+    assert (r2 == rows).all() and (v2 == vals).all()
+    v3, r3, _ = itcpd.sparse_sign(d, m, z, injective=injective, seed=8)
+    assert (r3 != rows).any()
+    # rows spread evenly: every row count within 6 standard deviations of its mean
+    counts = np.bincount(rows, minlength=d)
+    mean = m * z / d
+    assert np.abs(counts - mean).max() < 6 * math.sqrt(mean)
+    assert abs(float((vals > 0).mean()) - 0.5) < 0.05
+    # zeta beyond d is clamped to d: every row once per column
+    v4, r4, _ = itcpd.sparse_sign(4, 10, 9, seed=1)
+    assert P.sparse_sign_structure_ok(4, 10, 4, v4, r4, False)
+
+
+def test_sketch_pivots_and_seqrcs_match_oracle(itcpd):
+    rng = np.random.default_rng(4)
+    T = rng.standard_normal((20, 15, 12))
+    with itcpd.ALSPlan(T, 2) as plan:
+        for n in range(3):
+            m = T.shape[n]
+            ncol = T.size // m
+            l, s = int(round(3 * m * math.log(m))), int(round(math.log(m)))
+            vals, rows, _ = itcpd.sparse_sign(l, ncol, s, seed=11 + n)
+            p, r = plan.sketch_qr_pivots(n, l, s, rows, vals)
+            A_sk = P.sketched_matricization(T, n, l, rows, vals, s)
+            p0, r0 = P.qrcp(A_sk)
+            assert (p == p0).all() and np.abs(r - r0).max() <= 1e-12 * np.abs(r0).max()
+            q, rq = plan.seqrcs(n, l, s, 25, rows, vals)
+            q0, rq0 = P.seqrcs(T, n, l, s, 25, rows, vals)
+            assert (q == q0).all() and np.abs(rq - rq0).max() <= 1e-12 * np.abs(rq0).max()
+        with pytest.raises(itcpd.ITCPDError):
+            bad = rows.copy()
+            bad[0] = l + 5
+            plan.sketch_qr_pivots(2, l, s, bad, vals)
+
+
+def test_projection_setup_matches_oracle(itcpd):
+    T = np.random.default_rng(5).standard_normal((10, 9, 8))
+    with itcpd.ALSPlan(T, 3) as plan:
+        for shuffle in (False, True):
+            st = plan.projection_setup(itcpd.QRPivProjected(2, 40), trunc_tol=0.05, shuffle_pivots=shuffle, rng=9)
+            smp, piv, meff = P.projection_samples(T, 2, 40, trunc_tol=0.05, shuffle_pivots=shuffle, rng=9)
+            assert st.effective_ranks == meff
+            for n in range(3):
+                assert (st.pivots[n] == piv[n]).all() and (st.samples[n] == smp[n]).all()
+        st2 = itcpd.update_samples(st, 60, reshuffle=False)
+        assert st2.alg.stop(0) == 60 and st2.alg.start(0) == 2
+        for n in range(3):
+            assert (st2.samples[n] == st.pivots[n][1:60]).all()
+        st3 = itcpd.update_samples(st, 50, reshuffle=True, rng=3)
+        for n in range(3):
+            assert (st3.pivots[n][:st.effective_ranks[n]] == st.pivots[n][:st.effective_ranks[n]]).all()
+
+
+@pytest.mark.parametrize("normal", [True, False])
+@pytest.mark.parametrize("check", ["diff", "fit"])
+def test_qrpiv_run_matches_oracle(itcpd, normal, check):
+    T = O.reconstruct(*[getattr(O.random_CPD(np.zeros((12, 10, 9)), 4, np.random.default_rng(6)), k)
+                        for k in ("factors", "lam")])
+    T = T + 1e-3 * np.random.default_rng(7).standard_normal(T.shape)
+    cp0 = O.random_CPD(T, 4, np.random.default_rng(8))
+    mk = {"diff": (lambda: itcpd.CPDiffCheck(1e-6, 15), lambda: O.CPDiffCheck(1e-6, 15)),
+          "fit": (lambda: itcpd.FitCheck(1e-6, 15, 1.0), lambda: O.FitCheck(1e-6, 15, 1.0))}[check]
+    cg, cc = mk[0](), mk[1]()
+    got = itcpd.als_optimize(T, itcpd.CPD(cp0.factors, cp0.lam), alg=itcpd.QRPivProjected(60), check=cg,
+                             normal=normal, rng=21)
+    smp, _, _ = P.projection_samples(T, 1, 60, rng=21)
+    ref = P.optimize_projected(T, cp0, smp, check=cc, normal=normal)
+    for a, b in zip(got.factors, ref.factors):
+        assert rel(a, b) < 1e-8
+    assert rel(got.lam, ref.lam) < 1e-8
+    assert cg.iter == cc.iter
+
+
+def test_reference_bars_rank_400(itcpd):
+    """rand_cp_als.jl:20-70: on a 20x30x40 Gaussian tensor with rank 400, 20 sweeps of
+    QRPivProjected(800) (normal and QR forms, trunc_tol 0.001) and of SEQRCSPivProjected
+    over every mode ((1, 800), ranks 100, no shuffle; ranks 20, injective, QR form)
+    land within 1e-2 / 1e-1 of the plain ALS reconstruction; update_samples keeps the
+    algorithm and its new range, and the effective rank of mode 1 with trunc_tol 4 is
+    below 20."""
+    A = np.random.default_rng(0).standard_normal((20, 30, 40))
+    cp0 = itcpd.random_CPD(A, 400, np.random.default_rng(1))
+    nrm = float(np.linalg.norm(A))
+    opt = itcpd.als_optimize(A, cp0, check=itcpd.FitCheck(1e-6, 20, nrm))
+    X = O.reconstruct(opt.factors, opt.lam)
+
+    st = itcpd.compute_projection(A, cp0, itcpd.QRPivProjected(800), trunc_tol=4, rng=1)
+    st = itcpd.update_samples(st, 900, reshuffle=False)
+    assert st.alg.stop(0) == 900 and st.alg.start(0) == 1 and type(st.alg) is itcpd.QRPivProjected
+    assert st.effective_ranks[0] < 20
+    itcpd.optimize_projected(A, cp0, st, itcpd.FitCheck(1e-6, 20, nrm))
+
+    for normal in (True, False):
+        got = itcpd.als_optimize(A, cp0, alg=itcpd.QRPivProjected(800), check=itcpd.FitCheck(1e-6, 20, nrm),
+                                 normal=normal, shuffle_pivots=True, trunc_tol=0.001, rng=3)
+        assert rel(O.reconstruct(got.factors, got.lam), X) < 1e-2
+
+    alg = itcpd.SEQRCSPivProjected(1, 800, (0, 1, 2), (100, 100, 100))
+    st = itcpd.compute_projection(A, cp0, alg, rng=4, seed=40)
+    st = itcpd.update_samples(st, 600, reshuffle=True, rng=5)
+    assert st.alg.stop(0) == 600 and type(st.alg) is itcpd.SEQRCSPivProjected
+    itcpd.optimize_projected(A, cp0, st, itcpd.FitCheck(1e-6, 20, nrm))
+    got = itcpd.als_optimize(A, cp0, alg=itcpd.SEQRCSPivProjected((1,), (800,), (0, 1, 2), (100, 100, 100)),
+                             check=itcpd.FitCheck(1e-6, 20, nrm), shuffle_pivots=False, rng=4, seed=40)
+    assert rel(O.reconstruct(got.factors, got.lam), X) < 1e-1
+    got = itcpd.als_optimize(A, cp0, alg=itcpd.SEQRCSPivProjected((1,), (800,), (0, 1, 2), (20, 20, 20)),
+                             check=itcpd.FitCheck(1e-6, 20, nrm), shuffle_pivots=True, normal=False,
+                             injective=True, rng=6, seed=60)
+    assert rel(O.reconstruct(got.factors, got.lam), X) < 1e-1
+
+
+def test_reference_bar_interpolates_low_rank(itcpd):
+    """rand_cp_als.jl:73-83: a rank-20 target on 20x30x40, rank-10 runs with
+    CPDiffCheck(1e-5, 100): QRPivProjected(1200) reaches the plain ALS error within 10%."""
+    cpT = O.random_CPD(np.zeros((20, 30, 40)), 20, np.random.default_rng(21))
+    A = O.reconstruct(cpT.factors, cpT.lam)
+    cp0 = itcpd.random_CPD(A, 10, np.random.default_rng(22))
+    opt = itcpd.als_optimize(A, cp0, check=itcpd.CPDiffCheck(1e-5, 100))
+    exact = rel(O.reconstruct(opt.factors, opt.lam), A)
+    got = itcpd.als_optimize(A, cp0, alg=itcpd.QRPivProjected(1200), check=itcpd.CPDiffCheck(1e-5, 100), rng=2)
+    err = rel(O.reconstruct(got.factors, got.lam), A)
+    assert abs(exact - err) / exact < 0.1
