@@ -46,6 +46,11 @@ typedef enum { ITCPD_FP64 = 0, ITCPD_FP32 = 1, ITCPD_BF16X3 = 2 } itcpd_prec;
 typedef enum { ITCPD_ALG_KRP_FREE_NORMAL = 0, ITCPD_ALG_KRP_NORMAL = 1 } itcpd_alg;
 
 typedef enum { ITCPD_MEM_HOST = 0, ITCPD_MEM_DEVICE = 1 } itcpd_mem;
+/* Matrix whose column-pivoted QR itcpd_qr_pivots / itcpd_sketch_qr_pivots take: the
+ * tensor unfolding T_(mode) (QRPivProjected, SEQRCSPivProjected) or the transposed
+ * Khatri-Rao product of the plan's current factors other than `mode`
+ * (KSEQRCSPivProjected, qr_lev_score_sampled.jl:250-262). */
+typedef enum { ITCPD_PIVOT_TENSOR = 0, ITCPD_PIVOT_KRP = 1 } itcpd_pivot_source;
 
 /* NoCheck, FitCheck, CPDiffCheck (converge_checks/cp_diff_check.jl:6-71) and
  * CPAngleCheck (cp_angle_check.jl:6-73). */
@@ -192,20 +197,24 @@ int itcpd_sampled_solve(itcpd_plan* plan, int32_t mode, int64_t nsamps, const in
  * (optimizers/als_optimizers/randomized/qr_lev_score_sampled.jl:23-24) and, for a
  * column subset, in SEQRCS (SEQRCS.jl:160).  Tmat = reshape(array(T, (i_mode,
  * others...)), (I_mode, prod others)): column c enumerates the other modes with the
- * first one fastest.  cols (host, ncols entries, 0-based; NULL: every column, and
- * ncols must then be prod of the other dims) restricts the QR to those columns in
- * that order.  jpvt (host, ncols) = the column order as 0-based positions into the
- * list, rdiag (host, min(I_mode, ncols)) = the diagonal of R.  A set-up step before
- * the sweeps (one workgroup on the device); synchronous. */
-int itcpd_qr_pivots(itcpd_plan* plan, int32_t mode, const int64_t* cols, int64_t ncols,
-                    int64_t* jpvt, double* rdiag);
+ * first one fastest.  With source ITCPD_PIVOT_KRP the matrix is instead the
+ * R x prod(others) transposed Khatri-Rao product of the current factors
+ * (qr_lev_score_sampled.jl:259-261; SEQRCS.jl:217-218 for a subset).  cols (host,
+ * ncols entries, 0-based; NULL: every column, and ncols must then be prod of the
+ * other dims) restricts the QR to those columns in that order.  jpvt (host, ncols) =
+ * the column order as 0-based positions into the list, rdiag (host, min(rows,
+ * ncols)) = the diagonal of R.  A set-up step before the sweeps (one workgroup on the
+ * device); synchronous. */
+int itcpd_qr_pivots(itcpd_plan* plan, int32_t mode, int32_t source, const int64_t* cols,
+                    int64_t ncols, int64_t* jpvt, double* rdiag);
 
 /* The same for the sketch T_(mode) Omega^T of SEQRCS (SEQRCS.jl:140-146:
- * sketched_matricization, pivot_mapping.jl:103-139, then qr!(A_sk, ColumnNorm())).
- * Omega (l x prod others) is given as s row indices (0-based, < l) and s values per
- * column (host, column after column, as itcpd_sparse_sign writes them).  jpvt (host,
- * l), rdiag (host, min(I_mode, l)). */
-int itcpd_sketch_qr_pivots(itcpd_plan* plan, int32_t mode, int64_t l, int32_t s,
+ * sketched_matricization, pivot_mapping.jl:103-139, then qr!(A_sk, ColumnNorm())), or
+ * with ITCPD_PIVOT_KRP for (Omega KRP)^T (SEQRCS.jl:194-199, omega_hadamard,
+ * had_contract.jl:300-329).  Omega (l x prod others) is given as s row indices
+ * (0-based, < l) and s values per column (host, column after column, as
+ * itcpd_sparse_sign writes them).  jpvt (host, l), rdiag (host, min(rows, l)). */
+int itcpd_sketch_qr_pivots(itcpd_plan* plan, int32_t mode, int32_t source, int64_t l, int32_t s,
                            const int32_t* rows, const double* vals, int64_t* jpvt, double* rdiag);
 
 /* Sparse sign embedding d x m with zeta nonzeros per column, replacing the

@@ -11,7 +11,7 @@ a test written like the reference's ``test/cp_als.jl``) reads the same:
     CPAngleCheck(tol, max), CPDFit(check)                      converge_checks/*.jl
     KRPFreeNormal(), KRPNormal()                               algorithms/.../standard/tensor.jl
     LevScoreSampled(n); als_optimize(...; normal, stop_resample) randomized/krp_lev_score_sampled.jl
-    QRPivProjected(n), SEQRCSPivProjected(s, e, modes, ranks);
+    QRPivProjected(n), SEQRCSPivProjected(s, e, modes, ranks), KSEQRCSPivProjected(...);
     als_optimize(...; shuffle_pivots, trunc_tol, injective),
     update_samples                                          randomized/qr_lev_score_sampled.jl
     CPD (factors, lam), cp_rank                                cpd.jl:7-46
@@ -21,7 +21,7 @@ All arithmetic runs in libitcpd_hip.so on the GPU; there is no CPU fallback.
 from .api import (  # noqa: F401
     CPD, ALSPlan, BatchALS, DeviceTensor, MLP, reconstruct, CPDFit, FitCheck, CPDiffCheck, CPAngleCheck, KRPFreeNormal, KRPNormal, NoCheck, CPDOptimizer,
     LevScoreSampled, BlockLevScoreSampled, ProjectionAlgorithm, QRPivProjected, SEQRCSPivProjected,
-    PivotSetup, compute_projection, optimize_projected, update_samples, sparse_sign,
+    KSEQRCSPivProjected, PivotSetup, compute_projection, optimize_projected, update_samples, sparse_sign,
     als_optimize, cp_rank, decompose, decompose_adaptive, decompose_batch, increase_cpd_rank,
     random_CPD, tensor_norm,
 )
@@ -30,7 +30,7 @@ from ._lib import ITCPDError  # noqa: F401
 __all__ = [
     "CPD", "ALSPlan", "BatchALS", "decompose_batch", "DeviceTensor", "MLP", "reconstruct", "CPDFit", "FitCheck", "CPDiffCheck", "CPAngleCheck", "KRPFreeNormal", "KRPNormal", "NoCheck",
     "LevScoreSampled", "BlockLevScoreSampled", "ProjectionAlgorithm", "QRPivProjected",
-    "SEQRCSPivProjected", "PivotSetup", "compute_projection", "optimize_projected", "update_samples",
+    "SEQRCSPivProjected", "KSEQRCSPivProjected", "PivotSetup", "compute_projection", "optimize_projected", "update_samples",
     "sparse_sign", "CPDOptimizer", "als_optimize", "cp_rank", "decompose", "decompose_adaptive",
     "increase_cpd_rank", "random_CPD", "tensor_norm", "ITCPDError",
 ]

@@ -17,6 +17,7 @@ ITCPD_E_UNSUPPORTED = -6
 ITCPD_FP64, ITCPD_FP32, ITCPD_BF16X3 = 0, 1, 2
 ITCPD_ALG_KRP_FREE_NORMAL, ITCPD_ALG_KRP_NORMAL = 0, 1
 ITCPD_MEM_HOST, ITCPD_MEM_DEVICE = 0, 1
+ITCPD_PIVOT_TENSOR, ITCPD_PIVOT_KRP = 0, 1
 ITCPD_CHECK_NONE, ITCPD_CHECK_FIT, ITCPD_CHECK_CPDIFF, ITCPD_CHECK_CPANGLE = 0, 1, 2, 3
 ITCPD_KERNEL_GEMM_LEFT, ITCPD_KERNEL_GEMM_RIGHT, ITCPD_KERNEL_GEMM_ALL = 0, 1, 2
 
@@ -63,8 +64,8 @@ SIGNATURES = {
     "itcpd_sampled_solve": (C.c_int, [_P, C.c_int32, C.c_int64, _P, C.c_int32, C.c_int32,
                                       C.POINTER(C.c_int32)]),
     "itcpd_check_cpd": (C.c_int, [_P, C.POINTER(CheckStruct)]),
-    "itcpd_qr_pivots": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, _P]),
-    "itcpd_sketch_qr_pivots": (C.c_int, [_P, C.c_int32, C.c_int64, C.c_int32, _P, _P, _P, _P]),
+    "itcpd_qr_pivots": (C.c_int, [_P, C.c_int32, C.c_int32, _P, C.c_int64, _P, _P]),
+    "itcpd_sketch_qr_pivots": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int64, C.c_int32, _P, _P, _P, _P]),
     "itcpd_sparse_sign": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_uint64, _P, _P, _P,
                                     C.c_int32]),
     "itcpd_plan_set_precision": (C.c_int, [_P, C.c_int32]),

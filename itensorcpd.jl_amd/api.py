@@ -108,8 +108,17 @@ class SEQRCSPivProjected(QRPivProjected):
         super().__init__(start, end)
         self.random_modes = None if random_modes is None else tuple(int(m) for m in random_modes)
         if rank_vect is not None and not isinstance(rank_vect, dict):
-            rank_vect = dict(zip(self.random_modes, (int(r) for r in rank_vect)))
+            # Dict(rrmodes .=> Tuple(rank_vect)): one value broadcasts over the modes
+            rv = (int(rank_vect),) if np.isscalar(rank_vect) else tuple(int(r) for r in rank_vect)
+            rank_vect = {m: rv[0] if len(rv) == 1 else rv[q] for q, m in enumerate(self.random_modes)}
         self.rank_vect = rank_vect
+
+
+class KSEQRCSPivProjected(SEQRCSPivProjected):
+    """qr_lev_score_sampled.jl:55-84: as SEQRCSPivProjected, but the pivots come from the
+    Khatri-Rao product of a preliminary CPD (10 rank-sized LevScoreSampled sweeps with
+    the first draws kept, compute_als :176-194) instead of from T: a column-pivoted QR of
+    KRP^T for the other modes, SE-QRCS on (Omega KRP)^T for random_modes (0-based)."""
 
 
 class PivotSetup:
@@ -137,6 +146,12 @@ def _pivot_range(alg, mode: int, ncols: int):
     if not (0 < b <= e):
         raise ValueError("sample range start must lie in 1..end")
     return b, e
+
+
+def _pivot_source(source: str) -> int:
+    if source not in ("tensor", "krp"):
+        raise ValueError("pivot source must be 'tensor' or 'krp'")
+    return L.ITCPD_PIVOT_KRP if source == "krp" else L.ITCPD_PIVOT_TENSOR
 
 
 def _column_to_multi_coords(cols, dims) -> np.ndarray:
@@ -638,10 +653,11 @@ class ALSPlan:
         return sweeps
 
     # -- pivot-based projections (QRPivProjected, SEQRCSPivProjected)
-    def qr_pivots(self, mode: int, cols=None):
-        """Column-pivoted QR (qr(A, ColumnNorm())) of the mode-`mode` unfolding, or of the
-        listed 0-based columns of it.  Returns (column order as 0-based positions into
-        the list, diagonal of R)."""
+    def qr_pivots(self, mode: int, cols=None, source: str = "tensor"):
+        """Column-pivoted QR (qr(A, ColumnNorm())) of the mode-`mode` unfolding (source
+        "krp": of the transposed Khatri-Rao product of the current factors other than
+        `mode`), or of the listed 0-based columns of it.  Returns (column order as
+        0-based positions into the list, diagonal of R)."""
         nall = int(np.prod([d for m, d in enumerate(self.dims) if m != mode]))
         if cols is None:
             n, cptr = nall, None
@@ -649,47 +665,58 @@ class ALSPlan:
             c = np.ascontiguousarray(cols, dtype=np.int64)
             n, cptr = c.size, C.c_void_p(c.ctypes.data)
         jpvt = np.empty(n, dtype=np.int64)
-        rdiag = np.empty(min(self.dims[mode], n))
-        L.check(self._lib.itcpd_qr_pivots(self._h, mode, cptr, n, C.c_void_p(jpvt.ctypes.data),
-                                          C.c_void_p(rdiag.ctypes.data)))
+        rows = self.rank if source == "krp" else self.dims[mode]
+        rdiag = np.empty(min(rows, n))
+        L.check(self._lib.itcpd_qr_pivots(self._h, mode, _pivot_source(source), cptr, n,
+                                          C.c_void_p(jpvt.ctypes.data), C.c_void_p(rdiag.ctypes.data)))
         return jpvt, rdiag
 
-    def sketch_qr_pivots(self, mode: int, l: int, s: int, rows, vals):
-        """Column-pivoted QR of the sketch T_(mode) Omega^T (Omega: l rows, s nonzeros per
-        column given as 0-based rows and values, column after column)."""
+    def sketch_qr_pivots(self, mode: int, l: int, s: int, rows, vals, source: str = "tensor"):
+        """Column-pivoted QR of the sketch T_(mode) Omega^T (source "krp": (Omega KRP)^T)
+        with Omega of l rows, s nonzeros per column given as 0-based rows and values,
+        column after column."""
         r = np.ascontiguousarray(rows, dtype=np.int32)
         v = np.ascontiguousarray(vals, dtype=np.float64)
         nall = int(np.prod([d for m, d in enumerate(self.dims) if m != mode]))
         if r.size != nall * s or v.size != nall * s:
             raise ValueError("the embedding needs s entries per column of the unfolding")
         jpvt = np.empty(l, dtype=np.int64)
-        rdiag = np.empty(min(self.dims[mode], l))
-        L.check(self._lib.itcpd_sketch_qr_pivots(self._h, mode, int(l), int(s), C.c_void_p(r.ctypes.data),
+        rdiag = np.empty(min(self.rank if source == "krp" else self.dims[mode], l))
+        L.check(self._lib.itcpd_sketch_qr_pivots(self._h, mode, _pivot_source(source), int(l), int(s),
+                                                 C.c_void_p(r.ctypes.data),
                                                  C.c_void_p(v.ctypes.data), C.c_void_p(jpvt.ctypes.data),
                                                  C.c_void_p(rdiag.ctypes.data)))
         return jpvt, rdiag
 
-    def seqrcs(self, mode: int, l: int, s: int, t: int, rows, vals):
-        """SEQRCS(::Val{false}, ...) (SEQRCS.jl:127-175) with a given embedding: QR-CP of
-        the sketch, its first t pivots mapped back to the columns that touch those sketch
-        rows (pivot by pivot, increasing column, first occurrence kept), QR-CP of that
-        column subset.  Returns (pivots over every column, diagonal of the subset's R)."""
+    def seqrcs(self, mode: int, l: int, s: int, t: int, rows, vals, source: str = "tensor"):
+        """SEQRCS with a given embedding: QR-CP of the sketch, its first t pivots mapped
+        back to the columns of Omega with a nonzero in those sketch rows, QR-CP of that
+        column subset; pivots = the subset in that order, then the untouched columns.
+        On T (SEQRCS(::Val{false}, ...), SEQRCS.jl:127-175) the subset is built pivot by
+        pivot in increasing column, first occurrence kept; on the Khatri-Rao product
+        (SEQRCS(krp::Vector{ITensor}, ...), :178-240) in increasing column.  Returns
+        (pivots over every column, diagonal of the subset's R)."""
         nall = int(np.prod([d for m, d in enumerate(self.dims) if m != mode]))
-        p_sk, _ = self.sketch_qr_pivots(mode, l, s, rows, vals)
+        p_sk, _ = self.sketch_qr_pivots(mode, l, s, rows, vals, source)
         rmat = np.asarray(rows, dtype=np.int64).reshape(nall, s)
-        seen = np.zeros(nall, dtype=bool)
-        indices = []
-        for p in p_sk[:t]:
-            for c in np.nonzero((rmat == p).any(axis=1))[0]:
-                if not seen[c]:
-                    seen[c] = True
-                    indices.append(int(c))
-        indices = np.array(indices, dtype=np.int64)
-        p_sub, rdiag = self.qr_pivots(mode, indices)
+        if source == "krp":
+            seen = np.isin(rmat, p_sk[:t]).any(axis=1)
+            indices = np.nonzero(seen)[0].astype(np.int64)
+        else:
+            seen = np.zeros(nall, dtype=bool)
+            indices = []
+            for p in p_sk[:t]:
+                for c in np.nonzero((rmat == p).any(axis=1))[0]:
+                    if not seen[c]:
+                        seen[c] = True
+                        indices.append(int(c))
+            indices = np.array(indices, dtype=np.int64)
+        p_sub, rdiag = self.qr_pivots(mode, indices, source)
         return np.concatenate([indices[p_sub], np.nonzero(~seen)[0]]), rdiag
 
     def projection_setup(self, alg: "QRPivProjected", trunc_tol: float = 0.01, shuffle_pivots: bool = True,
-                         normal: bool = True, injective: bool = False, rng=None, seed=None) -> "PivotSetup":
+                         normal: bool = True, injective: bool = False, rng=None, seed=None,
+                         prelim_samples=None, prelim_niter: int = 10) -> "PivotSetup":
         """compute_als(::QRPivProjected / ::SEQRCSPivProjected) (optimizers/als_optimizers/
         randomized/qr_lev_score_sampled.jl:1-174): per mode the pivot order (QR-CP of the
         unfolding, or SE-QRCS for alg.random_modes), the effective rank (|r_jj| / max >
@@ -700,6 +727,16 @@ class ALSPlan:
         rmodes = getattr(alg, "random_modes", None) or ()
         rank_sk = getattr(alg, "rank_vect", None)
         ref_pivs, pivots, meffs, samples = [], [], [], []
+        source, start_cpd, prelim = "tensor", None, None
+        if isinstance(alg, KSEQRCSPivProjected):
+            # compute_als(::KSEQRCSPivProjected) :176-194: 10 LevScoreSampled(10 R) sweeps
+            # from the current CPD, first draws kept; its factors give the pivots and the
+            # run itself starts from the current CPD again
+            start_cpd = self.get_cpd()
+            self.run_sampled(LevScoreSampled(prelim_samples or 10 * self.rank), NoCheck(prelim_niter), normal=True,
+                             stop_resample=0, rng=rng)
+            prelim = self.get_cpd()
+            source = "krp"
         for n in range(N):
             odims = [d for m, d in enumerate(self.dims) if m != n]
             nall = int(np.prod(odims))
@@ -711,9 +748,9 @@ class ALSPlan:
                 s_ = int(round(np.log(m)))
                 sd = int(rng.integers(0, 2**63)) if seed is None else int(seed) + n
                 vals, rows, _ = sparse_sign(l, nall, s_, injective, sd, self.device)
-                p, rdiag = self.seqrcs(n, l, min(s_, l), k_sk, rows, vals)
+                p, rdiag = self.seqrcs(n, l, min(s_, l), k_sk, rows, vals, source)
             else:
-                p, rdiag = self.qr_pivots(n)
+                p, rdiag = self.qr_pivots(n, None, source)
             ref_pivs.append(p.copy())
             a = np.abs(rdiag)
             meff = int((a / a.max() > trunc_tol).sum())
@@ -725,7 +762,11 @@ class ALSPlan:
             meffs.append(meff)
             samples.append(np.asfortranarray(coords[b - 1:e]))
         odl = [[d for m, d in enumerate(self.dims) if m != n] for n in range(N)]
-        return PivotSetup(alg, ref_pivs, pivots, meffs, samples, normal, odl)
+        if start_cpd is not None:
+            self.set_cpd(start_cpd)
+        out = PivotSetup(alg, ref_pivs, pivots, meffs, samples, normal, odl)
+        out.prelim_cpd = prelim
+        return out
 
     def run_projected(self, setup: "PivotSetup", check, verbose: bool = False) -> int:
         """optimize (optimize.jl:6-32) with the pivot-based project_krp / matricize_tensor
@@ -921,6 +962,7 @@ def compute_projection(target, cp: CPD, alg: QRPivProjected, trunc_tol: float = 
     """compute_als for a pivot-based projection on a fresh plan (see
     ALSPlan.projection_setup)."""
     with ALSPlan(target, cp.rank) as plan:
+        plan.set_cpd(cp)
         return plan.projection_setup(alg, trunc_tol=trunc_tol, shuffle_pivots=shuffle_pivots, normal=normal,
                                      injective=injective, rng=rng, seed=seed)
 

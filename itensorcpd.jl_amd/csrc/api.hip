@@ -699,10 +699,10 @@ int itcpd_sampled_solve(itcpd_plan* p, int32_t mode, int64_t nsamps, const int32
 
 // Column-pivoted QR of the unfolding (or a column subset) or of its sparse sketch;
 // the results come back to the host.
-static int qr_pivots_common(itcpd_plan* p, int32_t mode, const std::vector<int64_t>* cols, int64_t n,
+static int qr_pivots_common(itcpd_plan* p, int32_t mode, bool krp, const std::vector<int64_t>* cols, int64_t n,
                             const std::vector<int64_t>* rowptr, const std::vector<int64_t>* colidx,
                             const std::vector<double>* ovals, int64_t* jpvt, double* rdiag) {
-    const int64_t In = p->dims[mode];
+    const int64_t In = krp ? p->R : p->dims[mode];
     int rc;
     int64_t* d_cols = nullptr;
     int64_t* d_rowptr = nullptr;
@@ -735,7 +735,7 @@ static int qr_pivots_common(itcpd_plan* p, int32_t mode, const std::vector<int64
         if (!rc && !ovals->empty()) rc = copy_in(d_vals, ovals->data(), ovals->size() * 8, ITCPD_MEM_HOST, p->stream);
     }
     if (!rc)
-        rc = launch_qr_pivots(p, mode, d_cols, cols ? (int64_t)cols->size() : n, d_rowptr, d_colidx, d_vals,
+        rc = launch_qr_pivots(p, mode, krp, d_cols, cols ? (int64_t)cols->size() : n, d_rowptr, d_colidx, d_vals,
                               rowptr ? n : 0, d_jpvt, d_rdiag);
     std::vector<int32_t> hj((size_t)n);
     if (!rc) {
@@ -757,10 +757,17 @@ static int64_t other_columns(const itcpd_plan* p, int mode) {
     return c;
 }
 
-int itcpd_qr_pivots(itcpd_plan* p, int32_t mode, const int64_t* cols, int64_t ncols, int64_t* jpvt,
-                    double* rdiag) {
+static int check_source(const itcpd_plan* p, int32_t source) {
+    if (source != ITCPD_PIVOT_TENSOR && source != ITCPD_PIVOT_KRP) return fail(ITCPD_E_ARG, "unknown pivot source");
+    if (source == ITCPD_PIVOT_KRP && !p->have_cpd) return fail(ITCPD_E_STATE, "no CPD set");
+    return ITCPD_OK;
+}
+
+int itcpd_qr_pivots(itcpd_plan* p, int32_t mode, int32_t source, const int64_t* cols, int64_t ncols,
+                    int64_t* jpvt, double* rdiag) {
     if (!p || !jpvt || !rdiag) return fail(ITCPD_E_ARG, "null argument");
     if (mode < 0 || mode >= p->N) return fail(ITCPD_E_ARG, "mode out of range");
+    if (int rc = check_source(p, source)) return rc;
     const int64_t nall = other_columns(p, mode);
     if (ncols < 1 || ncols > (int64_t)INT32_MAX) return fail(ITCPD_E_ARG, "ncols out of range");
     if (!cols && ncols != nall) return fail(ITCPD_E_ARG, "without a column list ncols must be every column");
@@ -772,13 +779,15 @@ int itcpd_qr_pivots(itcpd_plan* p, int32_t mode, const int64_t* cols, int64_t nc
         for (int64_t c : hc)
             if (c < 0 || c >= nall) return fail(ITCPD_E_ARG, "column index out of range");
     }
-    return qr_pivots_common(p, mode, cols ? &hc : nullptr, ncols, nullptr, nullptr, nullptr, jpvt, rdiag);
+    return qr_pivots_common(p, mode, source == ITCPD_PIVOT_KRP, cols ? &hc : nullptr, ncols, nullptr, nullptr,
+                            nullptr, jpvt, rdiag);
 }
 
-int itcpd_sketch_qr_pivots(itcpd_plan* p, int32_t mode, int64_t l, int32_t s, const int32_t* rows,
+int itcpd_sketch_qr_pivots(itcpd_plan* p, int32_t mode, int32_t source, int64_t l, int32_t s, const int32_t* rows,
                            const double* vals, int64_t* jpvt, double* rdiag) {
     if (!p || !rows || !vals || !jpvt || !rdiag) return fail(ITCPD_E_ARG, "null argument");
     if (mode < 0 || mode >= p->N) return fail(ITCPD_E_ARG, "mode out of range");
+    if (int rc = check_source(p, source)) return rc;
     if (l < 1 || l > (int64_t)INT32_MAX || s < 1) return fail(ITCPD_E_ARG, "embedding size out of range");
     if ((p->dims[mode] > (int64_t)INT32_MAX)) return fail(ITCPD_E_UNSUPPORTED, "mode too long");
     const int64_t nall = other_columns(p, mode);
@@ -802,7 +811,7 @@ int itcpd_sketch_qr_pivots(itcpd_plan* p, int32_t mode, int64_t l, int32_t s, co
             ov[(size_t)at] = vals[e];
         }
     DeviceGuard dg(p->device);
-    return qr_pivots_common(p, mode, nullptr, l, &rowptr, &colidx, &ov, jpvt, rdiag);
+    return qr_pivots_common(p, mode, source == ITCPD_PIVOT_KRP, nullptr, l, &rowptr, &colidx, &ov, jpvt, rdiag);
 }
 
 int itcpd_sparse_sign(int32_t d, int64_t m, int32_t zeta, int32_t injective, uint64_t seed, double* vals,

@@ -184,9 +184,9 @@ int launch_col_sumsq(itcpd_plan* p, int mode, const double* X);
 int launch_leverage_scores(itcpd_plan* p, int mode, double* probs_dev, double** probs_used);
 int sampled_ws_prepare(itcpd_plan* p, int mode, int64_t ns);
 int launch_sampled_solve(itcpd_plan* p, int mode, int64_t ns, const int* samples, bool normal);
-int launch_qr_pivots(itcpd_plan* p, int mode, const int64_t* cols, int64_t ncols, const int64_t* omega_rowptr,
-                     const int64_t* omega_cols, const double* omega_vals, int64_t l, int* jpvt_dev,
-                     double* rdiag_dev);
+int launch_qr_pivots(itcpd_plan* p, int mode, bool krp, const int64_t* cols, int64_t ncols,
+                     const int64_t* omega_rowptr, const int64_t* omega_cols, const double* omega_vals, int64_t l,
+                     int* jpvt_dev, double* rdiag_dev);
 int launch_sparse_sign(int d, int64_t m, int zeta, bool injective, uint64_t seed, double* vals_dev, int* rows_dev,
                        hipStream_t st);
 

@@ -512,6 +512,48 @@ __global__ __launch_bounds__(256) void k_sketch(const TT* __restrict__ T, OtherM
     W[idx] = acc;
 }
 
+// The transposed Khatri-Rao product of the other modes' factors as the source (the
+// KSEQRCS pivots, qr_lev_score_sampled.jl:250-262): KRP^T(r, c) = prod_k A_k(i_k(c), r),
+// R rows.  W(r, q) = KRP^T(r, cols[q]) (cols null: column q), R x ncols.
+__global__ __launch_bounds__(256) void k_krp_cols(OtherModes om, int R, const int64_t* __restrict__ cols,
+                                                  int64_t ncols, double* __restrict__ W) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)R * ncols) return;
+    const int64_t q = idx / R;
+    const int r = (int)(idx - q * R);
+    int64_t c = cols ? cols[q] : q;
+    double v = 1.0;
+    for (int k = 0; k < om.n; ++k) {
+        v *= om.A[k][(c % om.dim[k]) + om.dim[k] * r];
+        c /= om.dim[k];
+    }
+    W[idx] = v;
+}
+
+// (Omega KRP)^T (omega_hadamard, had_contract.jl:300-329, transposed as SEQRCS.jl:199
+// takes it): W(r, j) = sum over the nonzeros e of row j of Omega (increasing column)
+// of vals[e] KRP(colidx[e], r), R x l.
+__global__ __launch_bounds__(256) void k_sketch_krp(OtherModes om, int R, const int64_t* __restrict__ rowptr,
+                                                    const int64_t* __restrict__ colidx,
+                                                    const double* __restrict__ vals, int64_t l,
+                                                    double* __restrict__ W) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)R * l) return;
+    const int64_t j = idx / R;
+    const int r = (int)(idx - j * R);
+    double acc = 0.0;
+    for (int64_t e = rowptr[j]; e < rowptr[j + 1]; ++e) {
+        int64_t c = colidx[e];
+        double v = 1.0;
+        for (int k = 0; k < om.n; ++k) {
+            v *= om.A[k][(c % om.dim[k]) + om.dim[k] * r];
+            c /= om.dim[k];
+        }
+        acc += vals[e] * v;
+    }
+    W[idx] = acc;
+}
+
 // Column-pivoted QR of W (m x n, ld m) in one workgroup (xGEQP3 / xLAQP2 pivots);
 // jpvt (n) = the column order, rdiag (min(m, n)) = the diagonal of R.
 template <int NT>
@@ -725,16 +767,17 @@ int sampled_ws_prepare(itcpd_plan* p, int mode, int64_t ns) {
     return ensure_sampled_ws(p, doubles, ints);
 }
 
-// Column-pivoted QR of T_(mode) restricted to the columns `cols` (device, ncols of
-// them; null: every column) or, with `omega_rows` given, of the sketch T_(mode)
-// Omega^T with Omega (l x ncols_all) by rows (omega_rowptr / omega_cols / omega_vals,
-// device).  Scratch is allocated here and freed after (a set-up step before the
+// Column-pivoted QR of T_(mode) (krp: of the transposed Khatri-Rao product of the
+// other modes' factors, R rows) restricted to the columns `cols` (device, ncols of
+// them; null: every column) or, with `omega_rowptr` given, of the sketch T_(mode)
+// Omega^T (krp: (Omega KRP)^T) with Omega (l x ncols_all) by rows (omega_rowptr /
+// omega_cols / omega_vals, device).  Scratch is allocated here and freed after (a set-up step before the
 // sweeps); jpvt_dev (n ints) and rdiag_dev (min(I_n, n)) receive the result, with n
 // = ncols or l.  The call synchronises the plan stream.
-int launch_qr_pivots(itcpd_plan* p, int mode, const int64_t* cols, int64_t ncols, const int64_t* omega_rowptr,
-                     const int64_t* omega_cols, const double* omega_vals, int64_t l, int* jpvt_dev,
-                     double* rdiag_dev) {
-    const int64_t In = p->dims[mode];
+int launch_qr_pivots(itcpd_plan* p, int mode, bool krp, const int64_t* cols, int64_t ncols,
+                     const int64_t* omega_rowptr, const int64_t* omega_cols, const double* omega_vals, int64_t l,
+                     int* jpvt_dev, double* rdiag_dev) {
+    const int64_t In = krp ? p->R : p->dims[mode];   // rows of the source
     const int64_t n = omega_rowptr ? l : ncols;
     const OtherModes om = other_modes(p, mode);
     const int64_t ms = tensor_stride(p, mode);
@@ -745,7 +788,12 @@ int launch_qr_pivots(itcpd_plan* p, int mode, const int64_t* cols, int64_t ncols
     double* tau = W + In * n;
     double* vwork = tau + std::min(In, n);
     const unsigned grid = (unsigned)cdiv(In * n, 256);
-    if (omega_rowptr) {
+    if (krp && omega_rowptr) {
+        hipLaunchKernelGGL(k_sketch_krp, dim3(grid), dim3(256), 0, p->stream, om, (int)In, omega_rowptr,
+                           omega_cols, omega_vals, l, W);
+    } else if (krp) {
+        hipLaunchKernelGGL(k_krp_cols, dim3(grid), dim3(256), 0, p->stream, om, (int)In, cols, ncols, W);
+    } else if (omega_rowptr) {
         if (p->elt_bytes == 8)
             hipLaunchKernelGGL(k_sketch<double>, dim3(grid), dim3(256), 0, p->stream, (const double*)p->T, om, ms,
                                In, omega_rowptr, omega_cols, omega_vals, l, W);

@@ -28,6 +28,9 @@ What pins each function (checks in ``tests/test_oracle_pivoted.py``):
                            blocks); the draws themselves come from libc rand() in the
                            reference and are not reproducible (parity unpinned for the
                            draws: the engine's generator is checked by the same invariants)
+  krp_columns /            pivot_hadamard rows at every multi-index (the explicit
+  omega_hadamard           Khatri-Rao product) and the dense product Omega KRP
+  seqrcs_krp               the map-back order against a brute-force scan
   optimize_projected       the reference's own bars (rand_cp_als.jl:40-48, 78-83, 166-178)
 """
 from __future__ import annotations
@@ -193,17 +196,60 @@ def seqrcs(T: np.ndarray, k: int, l: int, s: int, t: int, rows, vals):
     return np.concatenate([indices[p_sub], rem]), rdiag
 
 
+def krp_columns(factors, n: int, cols=None) -> np.ndarray:
+    """The transposed Khatri-Rao product of the factors other than n (R x prod others,
+    column c = the KRP row at c's multi-index, first other mode fastest), restricted to
+    `cols` (qr_lev_score_sampled.jl:259-261, SEQRCS.jl:217)."""
+    from .sampled import pivot_hadamard
+    others = [np.asarray(A) for m, A in enumerate(factors) if m != n]
+    odims = [A.shape[0] for A in others]
+    if cols is None:
+        cols = np.arange(int(np.prod(odims)))
+    return pivot_hadamard(others, column_to_multi_coords(cols, odims)).T.copy()
+
+
+def omega_hadamard(factors, n: int, l: int, rows, vals, s: int) -> np.ndarray:
+    """had_contract.jl:300-329: row j = sum over the nonzeros of row j of Omega (increasing
+    column c) of Omega(j, c) KRP(c, :) -> l x R."""
+    K = krp_columns(factors, n).T
+    ncol = K.shape[0]
+    rows = np.asarray(rows).reshape(ncol, s)
+    vals = np.asarray(vals).reshape(ncol, s)
+    out = np.zeros((l, K.shape[1]))
+    for j in range(l):
+        cs, qs = np.nonzero(rows == j)
+        if cs.size:
+            out[j] = vals[cs, qs] @ K[cs]
+    return out
+
+
+def seqrcs_krp(factors, n: int, l: int, s: int, t: int, rows, vals):
+    """SEQRCS(krp::Vector{ITensor}, ...) (SEQRCS.jl:178-240) for a given embedding:
+    QR-CP of (Omega KRP)^T, the columns of Omega with a nonzero in its first t pivot rows
+    (increasing), QR-CP of KRP^T on them; pivots = that subset in QR order, then the
+    rest increasing.  Returns (p, rdiag of the subset QR)."""
+    A_sk = omega_hadamard(factors, n, l, rows, vals, s)
+    p_sk, _ = qrcp(A_sk.T)
+    ncol = int(np.prod([np.asarray(A).shape[0] for m, A in enumerate(factors) if m != n]))
+    touched = np.isin(np.asarray(rows).reshape(ncol, s), p_sk[:t]).any(axis=1)
+    indices = np.nonzero(touched)[0]
+    p_sub, rdiag = qrcp(krp_columns(factors, n, indices))
+    return np.concatenate([indices[p_sub], np.nonzero(~touched)[0]]), rdiag
+
+
 def _range(v, n):
     return v if np.isscalar(v) else v[n]
 
 
 def projection_samples(T: np.ndarray, start, end, trunc_tol=0.01, shuffle_pivots=True,
-                       rng=None, random_modes=(), rank_vect=None, embeddings=None):
-    """compute_als(::QRPivProjected / ::SEQRCSPivProjected) (optimizers/.../
-    qr_lev_score_sampled.jl:1-174): per mode the pivot order, the effective rank, the
-    shuffled tail and the sample rows start..end (1-based, inclusive; end 0 = all).
-    ``embeddings[n] = (rows, vals)`` supplies the sparse sign embedding of a random mode
-    (the reference draws it from libc rand()).  Returns (samples, pivots, meffs)."""
+                       rng=None, random_modes=(), rank_vect=None, embeddings=None, krp_factors=None):
+    """compute_als(::QRPivProjected / ::SEQRCSPivProjected / ::KSEQRCSPivProjected)
+    (optimizers/.../qr_lev_score_sampled.jl:1-281): per mode the pivot order, the
+    effective rank, the shuffled tail and the sample rows start..end (1-based,
+    inclusive; end 0 = all).  ``embeddings[n] = (rows, vals)`` supplies the sparse sign
+    embedding of a random mode (the reference draws it from libc rand()).  With
+    ``krp_factors`` (KSEQRCS: the preliminary CPD's factors) the pivots come from their
+    Khatri-Rao product instead of T.  Returns (samples, pivots, meffs)."""
     rng = np.random.default_rng(rng)
     samples, pivots, meffs = [], [], []
     for n in range(T.ndim):
@@ -221,7 +267,12 @@ def projection_samples(T: np.ndarray, start, end, trunc_tol=0.01, shuffle_pivots
             l = int(round(3 * m * math.log(m)))
             s = int(round(math.log(m)))
             rows, vals = embeddings[n]
-            p, rdiag = seqrcs(T, n, l, s, k_sk, rows, vals)
+            if krp_factors is not None:
+                p, rdiag = seqrcs_krp(krp_factors, n, l, s, k_sk, rows, vals)
+            else:
+                p, rdiag = seqrcs(T, n, l, s, k_sk, rows, vals)
+        elif krp_factors is not None:
+            p, rdiag = qrcp(krp_columns(krp_factors, n))
         else:
             p, rdiag = qrcp(unfold(T, n))
         meff = effective_rank(rdiag, trunc_tol)

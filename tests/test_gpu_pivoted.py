@@ -1,8 +1,9 @@
-"""Pivot-based projected ALS (QRPivProjected, SEQRCSPivProjected) on the GPU against the
-oracle (oracle/pivoted.py): the column-pivoted QR of unfoldings, column subsets and
-sketches (the same pivots as LAPACK dgeqp3's xLAQP2 steps), the device sparse sign
-embedding's invariants, SE-QRCS's map-back, whole runs with the same samples, and the
-reference's bars (test/rand_cp_als.jl:20-70, 73-83)."""
+"""Pivot-based projected ALS (QRPivProjected, SEQRCSPivProjected, KSEQRCSPivProjected)
+on the GPU against the oracle (oracle/pivoted.py): the column-pivoted QR of unfoldings,
+column subsets, sketches and Khatri-Rao products (the same pivots as LAPACK dgeqp3's
+xLAQP2 steps), the device sparse sign embedding's invariants, SE-QRCS's map-back, the
+set-up and whole runs with the same samples, and the reference's bars
+(test/rand_cp_als.jl:20-95)."""
 import math
 
 import numpy as np
@@ -184,5 +185,81 @@ def test_reference_bar_interpolates_low_rank(itcpd):
     opt = itcpd.als_optimize(A, cp0, check=itcpd.CPDiffCheck(1e-5, 100))
     exact = rel(O.reconstruct(opt.factors, opt.lam), A)
     got = itcpd.als_optimize(A, cp0, alg=itcpd.QRPivProjected(1200), check=itcpd.CPDiffCheck(1e-5, 100), rng=2)
+    err = rel(O.reconstruct(got.factors, got.lam), A)
+    assert abs(exact - err) / exact < 0.1
+
+
+def test_krp_source_pivots_match_oracle(itcpd):
+    """The KSEQRCS sources: column-pivoted QR of KRP^T of the current factors, of a
+    column subset of it, and of (Omega KRP)^T; SE-QRCS on the Khatri-Rao product."""
+    rng = np.random.default_rng(14)
+    T = rng.standard_normal((16, 12, 10))
+    init = O.random_CPD(T, 6, np.random.default_rng(15))
+    with itcpd.ALSPlan(T, 6) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        for n in range(3):
+            p, r = plan.qr_pivots(n, source="krp")
+            p0, r0 = P.qrcp(P.krp_columns(init.factors, n))
+            assert (p[:6] == p0[:6]).all() and np.abs(r - r0).max() <= 1e-12 * np.abs(r0).max()
+            ncol = T.size // T.shape[n]
+            cols = rng.permutation(ncol)[:40]
+            p, r = plan.qr_pivots(n, cols, source="krp")
+            p0, r0 = P.qrcp(P.krp_columns(init.factors, n, cols))
+            assert (p[:6] == p0[:6]).all() and np.abs(r - r0).max() <= 1e-12 * np.abs(r0).max()
+            m = T.shape[n]
+            l, s = int(round(3 * m * math.log(m))), int(round(math.log(m)))
+            vals, rows, _ = itcpd.sparse_sign(l, ncol, s, seed=30 + n)
+            p, r = plan.sketch_qr_pivots(n, l, s, rows, vals, source="krp")
+            p0, r0 = P.qrcp(P.omega_hadamard(init.factors, n, l, rows, vals, s).T)
+            assert (p[:6] == p0[:6]).all() and np.abs(r - r0).max() <= 1e-12 * np.abs(r0).max()
+            q, rq = plan.seqrcs(n, l, s, 4, rows, vals, source="krp")
+            q0, rq0 = P.seqrcs_krp(init.factors, n, l, s, 4, rows, vals)
+            k = min(6, len(rq0))
+            assert (q[:k] == q0[:k]).all() and sorted(q.tolist()) == list(range(ncol))
+            assert np.abs(rq - rq0).max() <= 1e-12 * np.abs(rq0).max()
+
+
+def test_kseqrcs_setup_matches_oracle(itcpd):
+    """compute_als(::KSEQRCSPivProjected): the preliminary LevScoreSampled CPD's
+    Khatri-Rao products give the pivots (SE-QRCS for modes 0 and 2, plain QR-CP for
+    mode 1); without shuffling the pivots and samples equal the oracle's from the same
+    preliminary factors and embeddings, and the run starts from the caller's CPD."""
+    T = np.random.default_rng(16).standard_normal((14, 12, 10))
+    cp0 = O.random_CPD(T, 5, np.random.default_rng(17))
+    alg = itcpd.KSEQRCSPivProjected(1, 90, (0, 2), 4)
+    with itcpd.ALSPlan(T, 5) as plan:
+        plan.set_cpd(itcpd.CPD(cp0.factors, cp0.lam))
+        st = plan.projection_setup(alg, shuffle_pivots=False, rng=3, seed=50)
+        back = plan.get_cpd()
+    for a, b in zip(back.factors, cp0.factors):
+        assert np.abs(a - b).max() == 0.0
+    emb = {}
+    for n in (0, 2):
+        m = T.shape[n]
+        l, s = int(round(3 * m * math.log(m))), int(round(math.log(m)))
+        vals, rows, _ = itcpd.sparse_sign(l, T.size // m, s, seed=50 + n)
+        emb[n] = (rows, vals)
+    smp, piv, meff = P.projection_samples(T, 1, 90, shuffle_pivots=False, random_modes=(0, 2),
+                                          rank_vect={0: 4, 2: 4}, embeddings=emb,
+                                          krp_factors=st.prelim_cpd.factors)
+    assert st.effective_ranks == meff
+    for n in range(3):
+        k = meff[n]
+        assert (st.pivots[n][:k] == piv[n][:k]).all()
+        assert sorted(map(tuple, st.pivots[n].tolist())) == sorted(map(tuple, piv[n].tolist()))
+
+
+@pytest.mark.parametrize("normal,injective", [(True, False), (False, False), (False, True), (True, True)])
+def test_reference_bar_kseqrcs(itcpd, normal, injective):
+    """rand_cp_als.jl:85-95: KSEQRCSPivProjected((1), (1200,), (1,2,3), 5) on the rank-20
+    20x30x40 target at rank 10 with CPDiffCheck(1e-5, 100) reaches the plain ALS error
+    within 10% (normal and QR forms, injective embeddings or not)."""
+    cpT = O.random_CPD(np.zeros((20, 30, 40)), 20, np.random.default_rng(21))
+    A = O.reconstruct(cpT.factors, cpT.lam)
+    cp0 = itcpd.random_CPD(A, 10, np.random.default_rng(22))
+    opt = itcpd.als_optimize(A, cp0, check=itcpd.CPDiffCheck(1e-5, 100))
+    exact = rel(O.reconstruct(opt.factors, opt.lam), A)
+    got = itcpd.als_optimize(A, cp0, alg=itcpd.KSEQRCSPivProjected(1, (1200,), (0, 1, 2), 5),
+                             check=itcpd.CPDiffCheck(1e-5, 100), normal=normal, injective=injective, rng=7)
     err = rel(O.reconstruct(got.factors, got.lam), A)
     assert abs(exact - err) / exact < 0.1

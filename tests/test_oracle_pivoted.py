@@ -202,3 +202,40 @@ def test_reference_bar_qrpiv_interpolates_low_rank(normal):
     got = P.optimize_projected(T, cp0, smp, check=O.CPDiffCheck(1e-5, 100), normal=normal)
     err = np.linalg.norm(T - O.reconstruct(got.factors, got.lam)) / np.linalg.norm(T)
     assert abs(exact - err) / exact < 0.1
+
+
+def test_krp_columns_and_omega_hadamard():
+    """KRP^T columns are the Khatri-Rao rows at each multi-index (first other mode
+    fastest), and omega_hadamard (had_contract.jl:300-329) is the dense product."""
+    rng = np.random.default_rng(12)
+    fac = [rng.standard_normal((d, 3)) for d in (4, 5, 6)]
+    for n in range(3):
+        others = [f for m, f in enumerate(fac) if m != n]
+        K = P.krp_columns(fac, n)
+        a, b = others
+        for c in range(K.shape[1]):
+            i, j = c % a.shape[0], c // a.shape[0]
+            assert np.abs(K[:, c] - a[i] * b[j]).max() < 1e-15
+        ncol = K.shape[1]
+        l, s = 7, 2
+        rows, vals = _embedding(l, ncol, s, rng)
+        Om = np.zeros((l, ncol))
+        Om[rows.reshape(ncol, s), np.arange(ncol)[:, None]] = vals.reshape(ncol, s)
+        assert np.abs(P.omega_hadamard(fac, n, l, rows, vals, s) - Om @ K.T).max() < 1e-13
+
+
+def test_seqrcs_krp_map_back():
+    """SEQRCS on the Khatri-Rao product (SEQRCS.jl:178-240): the subset is every column
+    of Omega touching the first t sketch pivots, in increasing order; the rest follow."""
+    rng = np.random.default_rng(13)
+    fac = [rng.standard_normal((d, 4)) for d in (6, 5, 7)]
+    n, l, s, t = 0, 11, 2, 3
+    ncol = 35
+    rows, vals = _embedding(l, ncol, s, rng)
+    p, r = P.seqrcs_krp(fac, n, l, s, t, rows, vals)
+    p_sk, _ = P.qrcp(P.omega_hadamard(fac, n, l, rows, vals, s).T)
+    want = sorted({c for c in range(ncol) for q in p_sk[:t] if q in rows[c * s:(c + 1) * s]})
+    sub, r0 = P.qrcp(P.krp_columns(fac, n, want))
+    assert p[:len(want)].tolist() == [want[k] for k in sub]
+    assert p[len(want):].tolist() == sorted(set(range(ncol)) - set(want))
+    assert np.abs(r - r0).max() == 0.0
