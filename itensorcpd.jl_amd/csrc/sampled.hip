@@ -22,6 +22,9 @@ namespace itcpd {
 namespace {
 
 constexpr int SNT = 512;   // threads of the one-workgroup QR kernels
+// widest matrix whose column-pivoted QR runs in one workgroup; wider ones take a
+// step per three grid-wide launches
+constexpr int64_t QRCP_ONE_WG_MAX = 4096;
 constexpr double EPS64 = 2.220446049250313e-16;
 
 struct OtherModes {
@@ -567,6 +570,116 @@ __global__ __launch_bounds__(NT) void k_qrcp(double* __restrict__ W, int m, int 
     for (int j = threadIdx.x; j < kq; j += NT) rdiag[j] = W[j + (int64_t)m * j];
 }
 
+// The same column-pivoted QR for wide matrices, one step per three launches over the
+// whole chip.  Every per-column operation is the one-workgroup form's (a wave per
+// column, lanes over rows, wave_sum; the reflector through house<SNT>), so both forms
+// give the same bits.
+__global__ __launch_bounds__(256) void k_qrcp_norms(const double* __restrict__ W, int m, int n,
+                                                    double* __restrict__ vn1, double* __restrict__ vn2,
+                                                    int* __restrict__ jpvt) {
+    const int64_t j = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+    const int ln = threadIdx.x & 63;
+    if (j >= n) return;
+    const double* c = W + (int64_t)m * j;
+    double ss = 0.0;
+    for (int k = ln; k < m; k += 64) ss += c[k] * c[k];
+    ss = wave_sum(ss);
+    if (ln == 0) {
+        vn1[j] = sqrt(ss);
+        vn2[j] = sqrt(ss);
+        jpvt[j] = (int)j;
+    }
+}
+
+// first position of the largest |vn1| among [lo, n) within each block's chunk
+__global__ __launch_bounds__(256) void k_qrcp_argmax_part(const double* __restrict__ vn1, int lo, int n,
+                                                          int per_block, double* __restrict__ part_v,
+                                                          int* __restrict__ part_i) {
+    __shared__ double redv[256];
+    __shared__ int redi[256];
+    const int b0 = lo + blockIdx.x * per_block;
+    const int b1 = min(n, b0 + per_block);
+    const int r = block_argmax<256>(b0, b1, [&](int j) { return fabs(vn1[j]); }, redv, redi);
+    if (threadIdx.x == 0) {
+        part_v[blockIdx.x] = r < b1 ? fabs(vn1[r]) : -__builtin_inf();
+        part_i[blockIdx.x] = r < b1 ? r : n;
+    }
+}
+
+// step i, one workgroup: the pivot from the block partials (first index on ties), the
+// swap, and the reflector of column i (rows i..m-1)
+__global__ __launch_bounds__(SNT) void k_qrcp_pivot(double* __restrict__ W, int m, int n, int i,
+                                                    const double* __restrict__ part_v,
+                                                    const int* __restrict__ part_i, int nparts,
+                                                    double* __restrict__ vn1, double* __restrict__ vn2,
+                                                    int* __restrict__ jpvt, double* __restrict__ tau,
+                                                    double* __restrict__ rdiag) {
+    __shared__ double redv[SNT];
+    __shared__ int redi[SNT];
+    const int t = threadIdx.x;
+    const int64_t ld = m;
+    const int q = block_argmax<SNT>(0, nparts, [&](int b) { return part_v[b]; }, redv, redi);
+    // among equal maxima the partial with the smallest position wins: blocks cover
+    // increasing ranges, so the first block holding the maximum has it
+    const int p = (q < nparts) ? part_i[q] : n;
+    const int pvt = (p < n) ? p : i;
+    if (pvt != i) {
+        for (int k = t; k < m; k += SNT) {
+            const double a = W[k + ld * pvt];
+            W[k + ld * pvt] = W[k + ld * i];
+            W[k + ld * i] = a;
+        }
+        if (t == 0) {
+            const int x = jpvt[pvt]; jpvt[pvt] = jpvt[i]; jpvt[i] = x;
+            vn1[pvt] = vn1[i];
+            vn2[pvt] = vn2[i];
+        }
+    }
+    __syncthreads();
+    double* col = W + i + ld * i;
+    const double ti = house<SNT>(col, m - i, redv);
+    if (t == 0) {
+        tau[i] = ti;
+        rdiag[i] = col[0];
+    }
+}
+
+// step i: (I - tau v v^T) on every column j > i and its partial-norm update
+// (xLAQP2's downdate, recomputed below sqrt(eps)); a wave per column
+__global__ __launch_bounds__(256) void k_qrcp_apply(double* __restrict__ W, int m, int n, int i,
+                                                    const double* __restrict__ tau, double* __restrict__ vn1,
+                                                    double* __restrict__ vn2) {
+    const int64_t j = i + 1 + (((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6);
+    const int ln = threadIdx.x & 63;
+    if (j >= n) return;
+    const int64_t ld = m;
+    const int len = m - i;
+    const double* v = W + i + ld * i;
+    double* c = W + i + ld * j;
+    const double ti = tau[i];
+    if (ti != 0.0) {
+        double s = 0.0;
+        for (int k = ln; k < len; k += 64) s += (k == 0 ? 1.0 : v[k]) * c[k];
+        const double f = ti * wave_sum(s);
+        for (int k = ln; k < len; k += 64) c[k] -= f * (k == 0 ? 1.0 : v[k]);
+    }
+    const double tol3z = 1.0536712127723509e-08;   // sqrt(eps)
+    const double v1 = vn1[j];
+    if (v1 != 0.0) {
+        double temp = fabs(W[i + ld * j]) / v1;
+        temp = fmax(1.0 - temp * temp, 0.0);
+        const double q = v1 / vn2[j];
+        if (temp * q * q <= tol3z) {
+            double ss = 0.0;
+            for (int k = i + 1 + ln; k < m; k += 64) ss += W[k + ld * j] * W[k + ld * j];
+            ss = wave_sum(ss);
+            if (ln == 0) { vn1[j] = sqrt(ss); vn2[j] = vn1[j]; }
+        } else if (ln == 0) {
+            vn1[j] = v1 * sqrt(temp);
+        }
+    }
+}
+
 // splitmix64 (Steele, Lea, Flood 2014): the stream of the sparse sign generator.
 __device__ __forceinline__ uint64_t splitmix_next(uint64_t& s) {
     uint64_t z = (s += 0x9E3779B97F4A7C15ull);
@@ -809,10 +922,43 @@ int launch_qr_pivots(itcpd_plan* p, int mode, bool krp, const int64_t* cols, int
                                ms, In, cols, ncols, W);
     }
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) {
+    if (e == hipSuccess && n <= QRCP_ONE_WG_MAX) {
         hipLaunchKernelGGL(k_qrcp<SNT>, dim3(1), dim3(SNT), 0, p->stream, W, (int)In, (int)n, tau, vwork, jpvt_dev,
                            rdiag_dev);
         e = hipGetLastError();
+    } else if (e == hipSuccess) {
+        // wide: a step per three launches; block partials of the argmax in the W buffer's tail
+        const int mi = (int)In, ni = (int)n;
+        double* vn1 = vwork;
+        double* vn2 = vwork + n;
+        const int per_block = 4096;
+        const int nparts = (int)cdiv(n, per_block);
+        double* part_v = nullptr;
+        int* part_i = nullptr;
+        if ((rc = dmalloc(&part_v, (size_t)nparts)) || (rc = dmalloc(&part_i, (size_t)nparts))) {
+            dfree(part_v);
+            dfree(part_i);
+            dfree(W);
+            return rc;
+        }
+        hipLaunchKernelGGL(k_qrcp_norms, dim3((unsigned)cdiv(n, 4)), dim3(256), 0, p->stream, W, mi, ni, vn1, vn2,
+                           jpvt_dev);
+        const int kq = (int)std::min(In, n);
+        for (int i = 0; i < kq && e == hipSuccess; ++i) {
+            const int np = (int)cdiv(n - i, per_block);
+            hipLaunchKernelGGL(k_qrcp_argmax_part, dim3((unsigned)np), dim3(256), 0, p->stream, vn1, i, ni,
+                               per_block, part_v, part_i);
+            hipLaunchKernelGGL(k_qrcp_pivot, dim3(1), dim3(SNT), 0, p->stream, W, mi, ni, i, part_v, part_i, np,
+                               vn1, vn2, jpvt_dev, tau, rdiag_dev);
+            if (i + 1 < ni)
+                hipLaunchKernelGGL(k_qrcp_apply, dim3((unsigned)cdiv(n - i - 1, 4)), dim3(256), 0, p->stream, W,
+                                   mi, ni, i, tau, vn1, vn2);
+            e = hipGetLastError();
+        }
+        const hipError_t e3 = hipStreamSynchronize(p->stream);
+        dfree(part_v);
+        dfree(part_i);
+        if (e == hipSuccess) e = e3;
     }
     const hipError_t e2 = hipStreamSynchronize(p->stream);
     dfree(W);

@@ -20,8 +20,11 @@ def rel(a, b):
 
 
 @pytest.mark.parametrize("dims,dt", [((20, 30, 40), np.float64), ((6, 5, 4, 3), np.float64),
-                                     ((9, 7, 11), np.float32), ((64, 3, 5), np.float64)])
+                                     ((9, 7, 11), np.float32), ((64, 3, 5), np.float64),
+                                     ((24, 80, 75), np.float64)])
 def test_qr_pivots_match_oracle(itcpd, dims, dt):
+    """Unfoldings up to 4096 columns take the one-workgroup QR, wider ones (mode 0 of
+    24x80x75: 6000 columns) the grid-wide steps; both give the oracle's pivots."""
     T = np.random.default_rng(1).standard_normal(dims).astype(dt)
     T64 = T.astype(np.float64)
     with itcpd.ALSPlan(T, 2) as plan:
@@ -263,3 +266,15 @@ def test_reference_bar_kseqrcs(itcpd, normal, injective):
                              check=itcpd.CPDiffCheck(1e-5, 100), normal=normal, injective=injective, rng=7)
     err = rel(O.reconstruct(got.factors, got.lam), A)
     assert abs(exact - err) / exact < 0.1
+
+
+def test_wide_krp_pivots_match_oracle(itcpd):
+    """A Khatri-Rao source wider than one workgroup's QR (6 x 5600)."""
+    T = np.random.default_rng(18).standard_normal((10, 80, 70))
+    init = O.random_CPD(T, 6, np.random.default_rng(19))
+    with itcpd.ALSPlan(T, 6) as plan:
+        plan.set_cpd(itcpd.CPD(init.factors, init.lam))
+        p, r = plan.qr_pivots(0, source="krp")
+        p0, r0 = P.qrcp(P.krp_columns(init.factors, 0))
+        assert (p[:6] == p0[:6]).all() and np.abs(r - r0).max() <= 1e-12 * np.abs(r0).max()
+        assert sorted(p.tolist()) == list(range(5600))
