@@ -25,6 +25,8 @@ constexpr int SNT = 512;   // threads of the one-workgroup QR kernels
 // widest matrix whose column-pivoted QR runs in one workgroup; wider ones take a
 // step per three grid-wide launches
 constexpr int64_t QRCP_ONE_WG_MAX = 4096;
+// tallest sampled Khatri-Rao block whose dgelsy QR stays in one workgroup
+constexpr int64_t QRCP_GRID_MIN_ROWS = 1024;
 constexpr double EPS64 = 2.220446049250313e-16;
 
 struct OtherModes {
@@ -190,12 +192,14 @@ __device__ void geqp3(double* W, int m, int n, int64_t ld, bool pivot, double* t
 
 // qr(K, ColumnNorm()) \ B as LAPACK dgelsy, factor part (one workgroup): K (m x n) in
 // W is overwritten by the column-pivoted QR (tau, jpvt), the numerical rank goes to
-// info[0], and for rank < n the rk x n trapezoid is reduced to [T11 0] Z (ztau).
+// info[0], and for rank < n the rk x n trapezoid is reduced to [T11 0] Z (ztau).  With
+// `factored` the column-pivoted QR is already in W / tau / jpvt (the grid-wide steps).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_lsq_qr_factor(double* __restrict__ W, int m, int n,
                                                       double* __restrict__ tau, double* __restrict__ ztau,
                                                       double* __restrict__ vwork, int* __restrict__ jpvt,
-                                                      int* __restrict__ info, const DevCheck* __restrict__ chk) {
+                                                      int* __restrict__ info, int factored,
+                                                      const DevCheck* __restrict__ chk) {
     if (chk->done) return;
     __shared__ double redv[NT];
     __shared__ int redi[NT];
@@ -205,14 +209,28 @@ __global__ __launch_bounds__(NT) void k_lsq_qr_factor(double* __restrict__ W, in
     const int kq = m < n ? m : n;
     double* vn1 = vwork;
     double* vn2 = vwork + n;
-    geqp3<NT>(W, m, n, ld, true, tau, jpvt, vn1, vn2, redv, redi);
+    if (!factored) geqp3<NT>(W, m, n, ld, true, tau, jpvt, vn1, vn2, redv, redi);
 
-    // numerical rank: incremental condition estimation on R, rcond = min(m, n) eps
+    // numerical rank: incremental condition estimation on R, rcond = min(m, n) eps, by
+    // one thread in column order; for kq <= 64 R's triangle and the two estimate
+    // vectors are read from LDS rather than global memory
+    __shared__ double r_lds[64 * 64];
+    __shared__ double x_lds[2 * 64];
+    const bool small = kq <= 64;
+    if (small) {
+        for (int x = t; x < kq * kq; x += NT) {
+            const int a = x % kq, b = x / kq;
+            r_lds[x] = a <= b ? W[a + ld * b] : 0.0;
+        }
+    }
+    __syncthreads();
     if (t == 0) {
         const double rcond = (double)kq * EPS64;
-        double* xmin = vwork;
-        double* xmax = vwork + n;
-        double smax = fabs(W[0]);
+        const double* Rm = small ? r_lds : W;
+        const int64_t ldr = small ? kq : ld;
+        double* xmin = small ? x_lds : vwork;
+        double* xmax = small ? x_lds + 64 : vwork + n;
+        double smax = fabs(Rm[0]);
         double smin = smax;
         int rk = 0;
         if (kq > 0 && smax != 0.0) {
@@ -223,10 +241,10 @@ __global__ __launch_bounds__(NT) void k_lsq_qr_factor(double* __restrict__ W, in
                 const int i = rk;
                 double amin = 0.0, amax = 0.0;
                 for (int k = 0; k < i; ++k) {
-                    amin += xmin[k] * W[k + ld * i];
-                    amax += xmax[k] * W[k + ld * i];
+                    amin += xmin[k] * Rm[k + ldr * i];
+                    amax += xmax[k] * Rm[k + ldr * i];
                 }
-                const double gamma = W[i + ld * i];
+                const double gamma = Rm[i + ldr * i];
                 double sminpr, s1, c1, smaxpr, s2, c2;
                 laic1(2, amin, smin, gamma, &sminpr, &s1, &c1);
                 laic1(1, amax, smax, gamma, &smaxpr, &s2, &c2);
@@ -457,6 +475,93 @@ __global__ __launch_bounds__(256) void k_lev_rows(const double* __restrict__ A, 
     probs[i] = s / (double)n;
 }
 
+// The same for n <= 64 with the factor in LDS: k_gram_chol64 (one workgroup) runs
+// k_gram_chol_inv's Cholesky steps on an LDS copy of G and writes U (column-major,
+// zero below the diagonal); k_lev_rows64 solves U^T y = a_i per row (a lane per row,
+// U broadcast from LDS) and p(i) = ||y||^2 / n, the same a_i^T G^-1 a_i / n without
+// forming G^-1.
+constexpr int LEV64_LD = 65;
+template <int NT>
+__global__ __launch_bounds__(NT) void k_gram_chol64(const double* __restrict__ G, int n, double* __restrict__ U,
+                                                    int* __restrict__ need, const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    __shared__ double S[64 * LEV64_LD];
+    __shared__ double redv[NT];
+    __shared__ int s_bad;
+    const int t = threadIdx.x;
+    for (int x = t; x < n * n; x += NT) S[(x % n) * LEV64_LD + x / n] = G[x];
+    double dmax = 0.0;
+    for (int j = t; j < n; j += NT) dmax = fmax(dmax, G[j + (int64_t)n * j]);
+    redv[t] = dmax;
+    if (t == 0) s_bad = 0;
+    __syncthreads();
+    for (int o = NT / 2; o > 0; o >>= 1) {
+        if (t < o) redv[t] = fmax(redv[t], redv[t + o]);
+        __syncthreads();
+    }
+    const double thr = (double)n * EPS64 * redv[0];
+    for (int j = 0; j < n; ++j) {
+        const double d = S[j * LEV64_LD + j];
+        if (!(d > thr)) {
+            if (t == 0) s_bad = 1;
+            __syncthreads();
+            break;
+        }
+        const double ujj = sqrt(d), inv = 1.0 / ujj;
+        __syncthreads();
+        for (int k = j + 1 + t; k < n; k += NT) S[j * LEV64_LD + k] *= inv;
+        if (t == 0) S[j * LEV64_LD + j] = ujj;
+        __syncthreads();
+        const int r = n - j - 1;
+        for (int x = t; x < r * r; x += NT) {
+            const int a = j + 1 + x % r, b = j + 1 + x / r;
+            if (a <= b) S[a * LEV64_LD + b] -= S[j * LEV64_LD + a] * S[j * LEV64_LD + b];
+        }
+        __syncthreads();
+    }
+    if (s_bad) {
+        if (t == 0) *need = 1;
+        return;
+    }
+    for (int x = t; x < n * n; x += NT) {
+        const int a = x % n, b = x / n;
+        U[x] = a <= b ? S[a * LEV64_LD + b] : 0.0;
+    }
+    if (t == 0) *need = 0;
+}
+
+__global__ __launch_bounds__(64) void k_lev_rows64(const double* __restrict__ A, int m, int n,
+                                                   const double* __restrict__ U, const int* __restrict__ need,
+                                                   double* __restrict__ probs, const DevCheck* __restrict__ chk) {
+    if (chk->done || *need != 0) return;
+    __shared__ double S[64 * LEV64_LD];
+    __shared__ double dinv[64];
+    const int t = threadIdx.x;
+    for (int x = t; x < n * n; x += 64) S[(x % n) * LEV64_LD + x / n] = U[x];
+    __syncthreads();
+    if (t < n) dinv[t] = 1.0 / S[t * LEV64_LD + t];
+    __syncthreads();
+    const int i = blockIdx.x * 64 + t;
+    if (i >= m) return;
+    double y[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) y[k] = k < n ? A[i + (int64_t)m * k] : 0.0;
+    // U^T y = a by columns of U^T (rows of U): y_j /= U(j,j), y_k -= U(j,k) y_j
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+        if (j < n) {
+            y[j] *= dinv[j];
+#pragma unroll
+            for (int k = j + 1; k < 64; ++k)
+                if (k < n) y[k] -= S[j * LEV64_LD + k] * y[j];
+        }
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 64; ++k) s += y[k] * y[k];
+    probs[i] = s / (double)n;
+}
+
 // p(i) = 1 / m for m <= n (Q is square and orthogonal: every row has norm 1)
 __global__ __launch_bounds__(256) void k_fill_value(double* __restrict__ out, int m, double v,
                                                     const DevCheck* __restrict__ chk) {
@@ -572,8 +677,8 @@ __global__ __launch_bounds__(NT) void k_qrcp(double* __restrict__ W, int m, int 
 
 // The same column-pivoted QR for wide matrices, one step per three launches over the
 // whole chip.  Every per-column operation is the one-workgroup form's (a wave per
-// column, lanes over rows, wave_sum; the reflector through house<SNT>), so both forms
-// give the same bits.
+// column, lanes over rows, wave_sum; the reflector through house<SNT>); the pivot
+// choices are the same, the last bits may differ with the compiler's contractions.
 __global__ __launch_bounds__(256) void k_qrcp_norms(const double* __restrict__ W, int m, int n,
                                                     double* __restrict__ vn1, double* __restrict__ vn2,
                                                     int* __restrict__ jpvt) {
@@ -640,7 +745,7 @@ __global__ __launch_bounds__(SNT) void k_qrcp_pivot(double* __restrict__ W, int 
     const double ti = house<SNT>(col, m - i, redv);
     if (t == 0) {
         tau[i] = ti;
-        rdiag[i] = col[0];
+        if (rdiag) rdiag[i] = col[0];
     }
 }
 
@@ -738,6 +843,30 @@ __global__ __launch_bounds__(256) void k_sparse_sign(int d, int64_t m, int zeta,
     }
 }
 
+// The grid-wide column-pivoted QR of W (m x n, ld m): tau (min(m, n)), jpvt (n), vwork
+// (2 n: the partial norms), rdiag (nullable), part_v / part_i (cdiv(n, 4096) each).
+hipError_t launch_grid_geqp3(double* W, int m, int n, double* tau, double* vwork, int* jpvt, double* rdiag,
+                             double* part_v, int* part_i, hipStream_t st) {
+    constexpr int per_block = 4096;
+    double* vn1 = vwork;
+    double* vn2 = vwork + n;
+    hipLaunchKernelGGL(k_qrcp_norms, dim3((unsigned)cdiv(n, 4)), dim3(256), 0, st, W, m, n, vn1, vn2, jpvt);
+    hipError_t e = hipGetLastError();
+    const int kq = m < n ? m : n;
+    for (int i = 0; i < kq && e == hipSuccess; ++i) {
+        const int np = (int)cdiv(n - i, per_block);
+        hipLaunchKernelGGL(k_qrcp_argmax_part, dim3((unsigned)np), dim3(256), 0, st, vn1, i, n, per_block, part_v,
+                           part_i);
+        hipLaunchKernelGGL(k_qrcp_pivot, dim3(1), dim3(SNT), 0, st, W, m, n, i, part_v, part_i, np, vn1, vn2, jpvt,
+                           tau, rdiag);
+        if (i + 1 < n)
+            hipLaunchKernelGGL(k_qrcp_apply, dim3((unsigned)cdiv(n - i - 1, 4)), dim3(256), 0, st, W, m, n, i, tau,
+                               vn1, vn2);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
 OtherModes other_modes(const itcpd_plan* p, int mode) {
     OtherModes om{};
     int64_t stride = 1;
@@ -811,10 +940,17 @@ int launch_leverage_scores(itcpd_plan* p, int mode, double* probs_dev, double** 
     hipLaunchKernelGGL(k_gemm_tn, dim3((unsigned)cdiv((int64_t)n * n * 64, 256)), dim3(256), 0, p->stream,
                        p->A[mode], p->A[mode], (int64_t)m, n, n, G, p->chk);
     ITCPD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_gram_chol_inv<256>, dim3(1), dim3(256), 0, p->stream, G, n, Ui, Gi, need, p->chk);
-    ITCPD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_lev_rows, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, p->stream, p->A[mode], m, n, Gi,
-                       need, probs, p->chk);
+    if (n <= 64) {
+        hipLaunchKernelGGL(k_gram_chol64<256>, dim3(1), dim3(256), 0, p->stream, G, n, Ui, need, p->chk);
+        ITCPD_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_lev_rows64, dim3((unsigned)cdiv(m, 64)), dim3(64), 0, p->stream, p->A[mode], m, n, Ui,
+                           need, probs, p->chk);
+    } else {
+        hipLaunchKernelGGL(k_gram_chol_inv<256>, dim3(1), dim3(256), 0, p->stream, G, n, Ui, Gi, need, p->chk);
+        ITCPD_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_lev_rows, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, p->stream, p->A[mode], m, n, Gi,
+                           need, probs, p->chk);
+    }
     ITCPD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_lev_scores<SNT>, dim3(1), dim3(SNT), 0, p->stream, p->A[mode], m, n, W, tau, vwork,
                        p->smp_iws, probs, need, p->chk);
@@ -864,8 +1000,16 @@ int launch_sampled_solve(itcpd_plan* p, int mode, int64_t ns, const int* samples
         return launch_solve(p, mode, p->M, 1, In * R, false, false);
     }
     ITCPD_HIP(hipMemcpyAsync(W, Ks, (size_t)nk * 8, hipMemcpyDeviceToDevice, p->stream));
+    // tall samples: the column-pivoted QR as grid-wide steps (a wave per column instead
+    // of eight waves for all of them), then rank estimate and RZ in one workgroup
+    const bool grid = ns >= QRCP_GRID_MIN_ROWS;
+    if (grid) {
+        double* part_v = vwork + 2 * R;
+        int* part_i = info + 4;
+        ITCPD_HIP(launch_grid_geqp3(W, (int)ns, R, tau, vwork, jpvt, nullptr, part_v, part_i, p->stream));
+    }
     hipLaunchKernelGGL(k_lsq_qr_factor<SNT>, dim3(1), dim3(SNT), 0, p->stream, W, (int)ns, R, tau, ztau, vwork,
-                       jpvt, info, p->chk);
+                       jpvt, info, grid ? 1 : 0, p->chk);
     ITCPD_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_lsq_qr_apply, dim3((unsigned)cdiv(In, 4)), dim3(256), (size_t)4 * R * 8, p->stream, W,
                        (int)ns, R, B, In, p->X, tau, ztau, jpvt, info, p->chk);
@@ -875,8 +1019,9 @@ int launch_sampled_solve(itcpd_plan* p, int mode, int64_t ns, const int* samples
 
 int sampled_ws_prepare(itcpd_plan* p, int mode, int64_t ns) {
     const int64_t R = p->R, In = p->dims[mode];
-    const size_t doubles = (size_t)(2 * ns * R + ns * In + R * R + 4 * R);
-    const size_t ints = (size_t)(ns * (p->N - 1) + R + 4);
+    // + the argmax partials of the grid-wide QR (one per 4096 columns; R <= 2048 here)
+    const size_t doubles = (size_t)(2 * ns * R + ns * In + R * R + 4 * R + 8);
+    const size_t ints = (size_t)(ns * (p->N - 1) + R + 4 + 8);
     return ensure_sampled_ws(p, doubles, ints);
 }
 
@@ -927,12 +1072,8 @@ int launch_qr_pivots(itcpd_plan* p, int mode, bool krp, const int64_t* cols, int
                            rdiag_dev);
         e = hipGetLastError();
     } else if (e == hipSuccess) {
-        // wide: a step per three launches; block partials of the argmax in the W buffer's tail
-        const int mi = (int)In, ni = (int)n;
-        double* vn1 = vwork;
-        double* vn2 = vwork + n;
-        const int per_block = 4096;
-        const int nparts = (int)cdiv(n, per_block);
+        // wide: a step per three launches
+        const int nparts = (int)cdiv(n, 4096);
         double* part_v = nullptr;
         int* part_i = nullptr;
         if ((rc = dmalloc(&part_v, (size_t)nparts)) || (rc = dmalloc(&part_i, (size_t)nparts))) {
@@ -941,20 +1082,7 @@ int launch_qr_pivots(itcpd_plan* p, int mode, bool krp, const int64_t* cols, int
             dfree(W);
             return rc;
         }
-        hipLaunchKernelGGL(k_qrcp_norms, dim3((unsigned)cdiv(n, 4)), dim3(256), 0, p->stream, W, mi, ni, vn1, vn2,
-                           jpvt_dev);
-        const int kq = (int)std::min(In, n);
-        for (int i = 0; i < kq && e == hipSuccess; ++i) {
-            const int np = (int)cdiv(n - i, per_block);
-            hipLaunchKernelGGL(k_qrcp_argmax_part, dim3((unsigned)np), dim3(256), 0, p->stream, vn1, i, ni,
-                               per_block, part_v, part_i);
-            hipLaunchKernelGGL(k_qrcp_pivot, dim3(1), dim3(SNT), 0, p->stream, W, mi, ni, i, part_v, part_i, np,
-                               vn1, vn2, jpvt_dev, tau, rdiag_dev);
-            if (i + 1 < ni)
-                hipLaunchKernelGGL(k_qrcp_apply, dim3((unsigned)cdiv(n - i - 1, 4)), dim3(256), 0, p->stream, W,
-                                   mi, ni, i, tau, vn1, vn2);
-            e = hipGetLastError();
-        }
+        e = launch_grid_geqp3(W, (int)In, (int)n, tau, vwork, jpvt_dev, rdiag_dev, part_v, part_i, p->stream);
         const hipError_t e3 = hipStreamSynchronize(p->stream);
         dfree(part_v);
         dfree(part_i);

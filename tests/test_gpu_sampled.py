@@ -52,7 +52,8 @@ def test_leverage_scores_rank_deficient_factor(itcpd):
 
 
 STEP_CASES = [((12, 13, 3), 5, 100, np.float64), ((9, 8, 7), 6, 3, np.float64),
-              ((20, 18, 16, 5), 80, 400, np.float64), ((12, 13, 3), 5, 60, np.float32)]
+              ((20, 18, 16, 5), 80, 400, np.float64), ((12, 13, 3), 5, 60, np.float32),
+              ((40, 30, 20), 16, 1500, np.float64)]
 
 
 @pytest.mark.parametrize("normal", [True, False])
@@ -61,7 +62,7 @@ STEP_CASES = [((12, 13, 3), 5, 100, np.float64), ((9, 8, 7), 6, 3, np.float64),
 def test_sampled_step_matches_oracle(itcpd, dims, R, ns, dt, normal):
     """Same samples on both sides; each mode in turn, so later modes see the updated
     factors.  Wide samples (ns < R) exercise the normal form's QR fallback and the QR
-    form's minimum-norm solve."""
+    form's minimum-norm solve; 1500 samples the QR form's grid-wide factorisation."""
     rng = np.random.default_rng(3)
     T = rng.standard_normal(dims).astype(dt)
     T64 = T.astype(np.float64)
