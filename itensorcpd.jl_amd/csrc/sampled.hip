@@ -354,6 +354,86 @@ __global__ __launch_bounds__(256) void k_lsq_qr_apply(const double* __restrict__
     for (int k = ln; k < n; k += 64) X[c + nrhs * jpvt[k]] = y[k];
 }
 
+// The same with the right-hand side in registers for m <= 64 NQ (lane l holds rows
+// l + 64 q): Q^T b touches global memory only for the reflectors, which every wave
+// reads alike.  The arithmetic and its order are k_lsq_qr_apply's.
+template <int NQ>
+__global__ __launch_bounds__(256) void k_lsq_qr_apply_reg(const double* __restrict__ W, int m, int n,
+                                                          const double* __restrict__ B, int64_t nrhs,
+                                                          double* __restrict__ X,
+                                                          const double* __restrict__ tau,
+                                                          const double* __restrict__ ztau,
+                                                          const int* __restrict__ jpvt, const int* __restrict__ info,
+                                                          const DevCheck* __restrict__ chk) {
+    if (chk->done) return;
+    extern __shared__ double ylds[];
+    const int wave = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * 4 + wave;
+    if (c >= nrhs) return;
+    const int64_t ld = m;
+    const int kq = m < n ? m : n;
+    const int rk = info[0];
+    const double* b = B + ld * c;
+    double bv[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int r = ln + 64 * q;
+        bv[q] = r < m ? b[r] : 0.0;
+    }
+    for (int i = 0; i < kq; ++i) {
+        const double ti = tau[i];
+        if (ti == 0.0) continue;
+        const double* v = W + ld * i;
+        double vr[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int r = ln + 64 * q;
+            vr[q] = (r < m && r > i) ? v[r] : (r == i ? 1.0 : 0.0);
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int r = ln + 64 * q;
+            if (r < m && r >= i) s += vr[q] * bv[q];
+        }
+        const double f = ti * wave_sum(s);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int r = ln + 64 * q;
+            if (r < m && r >= i) bv[q] -= f * vr[q];
+        }
+    }
+    double* y = ylds + (size_t)wave * n;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int k = ln + 64 * q;
+        if (k < n) y[k] = (k < rk) ? bv[q] : 0.0;
+    }
+    for (int k = 64 * NQ + ln; k < n; k += 64) y[k] = 0.0;
+    __threadfence_block();
+    for (int j = rk - 1; j >= 0; --j) {
+        double s = 0.0;
+        for (int k = j + 1 + ln; k < rk; k += 64) s += W[j + ld * k] * y[k];
+        s = wave_sum(s);
+        if (ln == 0) y[j] = (y[j] - s) / W[j + ld * j];
+        __threadfence_block();
+    }
+    if (rk < n) {
+        for (int tq = 0; tq < rk; ++tq) {
+            const double tz = ztau[tq];
+            if (tz == 0.0) continue;
+            double part = 0.0;
+            for (int k = rk + ln; k < n; k += 64) part += W[tq + ld * k] * y[k];
+            const double f = tz * (wave_sum(part) + y[tq]);
+            __threadfence_block();
+            if (ln == 0) y[tq] -= f;
+            for (int k = rk + ln; k < n; k += 64) y[k] -= f * W[tq + ld * k];
+            __threadfence_block();
+        }
+    }
+    for (int k = ln; k < n; k += 64) X[c + nrhs * jpvt[k]] = y[k];
+}
+
 // compute_leverage_score_probabilitiy (probability.jl:3-10): Q = the thin Q of the
 // unpivoted Householder QR of A (m x n), p(i) = sum_j Q(i, j)^2 / min(m, n).  W is
 // an m x n scratch (ld m); Q is formed in place as LAPACK xORG2R does.  Runs only
@@ -1011,8 +1091,12 @@ int launch_sampled_solve(itcpd_plan* p, int mode, int64_t ns, const int* samples
     hipLaunchKernelGGL(k_lsq_qr_factor<SNT>, dim3(1), dim3(SNT), 0, p->stream, W, (int)ns, R, tau, ztau, vwork,
                        jpvt, info, grid ? 1 : 0, p->chk);
     ITCPD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_lsq_qr_apply, dim3((unsigned)cdiv(In, 4)), dim3(256), (size_t)4 * R * 8, p->stream, W,
-                       (int)ns, R, B, In, p->X, tau, ztau, jpvt, info, p->chk);
+    if (ns <= 64 * 32)
+        hipLaunchKernelGGL(k_lsq_qr_apply_reg<32>, dim3((unsigned)cdiv(In, 4)), dim3(256), (size_t)4 * R * 8,
+                           p->stream, W, (int)ns, R, B, In, p->X, tau, ztau, jpvt, info, p->chk);
+    else
+        hipLaunchKernelGGL(k_lsq_qr_apply, dim3((unsigned)cdiv(In, 4)), dim3(256), (size_t)4 * R * 8, p->stream, W,
+                           (int)ns, R, B, In, p->X, tau, ztau, jpvt, info, p->chk);
     ITCPD_HIP(hipGetLastError());
     return ITCPD_OK;
 }
