@@ -211,11 +211,11 @@ __global__ __launch_bounds__(NT) void k_lsq_qr_factor(double* __restrict__ W, in
     double* vn2 = vwork + n;
     if (!factored) geqp3<NT>(W, m, n, ld, true, tau, jpvt, vn1, vn2, redv, redi);
 
-    // numerical rank: incremental condition estimation on R, rcond = min(m, n) eps, by
-    // one thread in column order; for kq <= 64 R's triangle and the two estimate
-    // vectors are read from LDS rather than global memory
+    // numerical rank: incremental condition estimation on R, rcond = min(m, n) eps.  For
+    // kq <= 64 wave 0 runs it on an LDS copy of R's triangle, lane k holding entry k of
+    // both estimate vectors (the dot products as wave sums); above that thread 0 runs it
+    // in column order on global memory.
     __shared__ double r_lds[64 * 64];
-    __shared__ double x_lds[2 * 64];
     const bool small = kq <= 64;
     if (small) {
         for (int x = t; x < kq * kq; x += NT) {
@@ -224,13 +224,44 @@ __global__ __launch_bounds__(NT) void k_lsq_qr_factor(double* __restrict__ W, in
         }
     }
     __syncthreads();
-    if (t == 0) {
-        const double rcond = (double)kq * EPS64;
-        const double* Rm = small ? r_lds : W;
-        const int64_t ldr = small ? kq : ld;
-        double* xmin = small ? x_lds : vwork;
-        double* xmax = small ? x_lds + 64 : vwork + n;
-        double smax = fabs(Rm[0]);
+    const double rcond = (double)kq * EPS64;
+    if (small && t < 64) {
+        const int lane = t;
+        double xm = lane == 0 ? 1.0 : 0.0, xM = xm;
+        double smax = fabs(r_lds[0]);
+        double smin = smax;
+        int rk = 0;
+        if (kq > 0 && smax != 0.0) {
+            rk = 1;
+            while (rk < kq) {
+                const int i = rk;
+                const double rki = lane < i ? r_lds[lane + kq * i] : 0.0;
+                const double amin = wave_sum(xm * rki), amax = wave_sum(xM * rki);
+                const double gamma = r_lds[i + kq * i];
+                double sminpr, s1, c1, smaxpr, s2, c2;
+                laic1(2, amin, smin, gamma, &sminpr, &s1, &c1);
+                laic1(1, amax, smax, gamma, &smaxpr, &s2, &c2);
+                if (smaxpr * rcond > sminpr) break;
+                if (lane < i) {
+                    xm *= s1;
+                    xM *= s2;
+                } else if (lane == i) {
+                    xm = c1;
+                    xM = c2;
+                }
+                smin = sminpr;
+                smax = smaxpr;
+                ++rk;
+            }
+        }
+        if (lane == 0) {
+            s_rank = rk;
+            info[0] = rk;
+        }
+    } else if (!small && t == 0) {
+        double* xmin = vwork;
+        double* xmax = vwork + n;
+        double smax = fabs(W[0]);
         double smin = smax;
         int rk = 0;
         if (kq > 0 && smax != 0.0) {
@@ -241,10 +272,10 @@ __global__ __launch_bounds__(NT) void k_lsq_qr_factor(double* __restrict__ W, in
                 const int i = rk;
                 double amin = 0.0, amax = 0.0;
                 for (int k = 0; k < i; ++k) {
-                    amin += xmin[k] * Rm[k + ldr * i];
-                    amax += xmax[k] * Rm[k + ldr * i];
+                    amin += xmin[k] * W[k + ld * i];
+                    amax += xmax[k] * W[k + ld * i];
                 }
-                const double gamma = Rm[i + ldr * i];
+                const double gamma = W[i + ld * i];
                 double sminpr, s1, c1, smaxpr, s2, c2;
                 laic1(2, amin, smin, gamma, &sminpr, &s1, &c1);
                 laic1(1, amax, smax, gamma, &smaxpr, &s2, &c2);
