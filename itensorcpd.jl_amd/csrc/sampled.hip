@@ -834,10 +834,16 @@ __global__ __launch_bounds__(SNT) void k_qrcp_pivot(double* __restrict__ W, int 
     __shared__ int redi[SNT];
     const int t = threadIdx.x;
     const int64_t ld = m;
-    const int q = block_argmax<SNT>(0, nparts, [&](int b) { return part_v[b]; }, redv, redi);
-    // among equal maxima the partial with the smallest position wins: blocks cover
-    // increasing ranges, so the first block holding the maximum has it
-    const int p = (q < nparts) ? part_i[q] : n;
+    // nparts == 0: the columns are few enough to scan here; else among equal maxima the
+    // partial with the smallest position wins (blocks cover increasing ranges, so the
+    // first block holding the maximum has it)
+    int p;
+    if (nparts == 0) {
+        p = block_argmax<SNT>(i, n, [&](int j) { return fabs(vn1[j]); }, redv, redi);
+    } else {
+        const int q = block_argmax<SNT>(0, nparts, [&](int b) { return part_v[b]; }, redv, redi);
+        p = (q < nparts) ? part_i[q] : n;
+    }
     const int pvt = (p < n) ? p : i;
     if (pvt != i) {
         for (int k = t; k < m; k += SNT) {
@@ -965,9 +971,11 @@ hipError_t launch_grid_geqp3(double* W, int m, int n, double* tau, double* vwork
     hipError_t e = hipGetLastError();
     const int kq = m < n ? m : n;
     for (int i = 0; i < kq && e == hipSuccess; ++i) {
-        const int np = (int)cdiv(n - i, per_block);
-        hipLaunchKernelGGL(k_qrcp_argmax_part, dim3((unsigned)np), dim3(256), 0, st, vn1, i, n, per_block, part_v,
-                           part_i);
+        // up to one block of columns the pivot kernel scans them itself
+        const int np = n <= per_block ? 0 : (int)cdiv(n - i, per_block);
+        if (np > 0)
+            hipLaunchKernelGGL(k_qrcp_argmax_part, dim3((unsigned)np), dim3(256), 0, st, vn1, i, n, per_block,
+                               part_v, part_i);
         hipLaunchKernelGGL(k_qrcp_pivot, dim3(1), dim3(SNT), 0, st, W, m, n, i, part_v, part_i, np, vn1, vn2, jpvt,
                            tau, rdiag);
         if (i + 1 < n)
