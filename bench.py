@@ -381,8 +381,9 @@ def bench_warmstart(args, E) -> dict:
 def bench_sampled(args, E) -> dict:
     """Leverage-score-sampled ALS (LevScoreSampled, the QR form that compute_als defaults
     to, and the normal form) on the plain workload's 512^3 rank-64 tensor: sweeps/s with
-    the tensor resident, next to the plain normal-equation sweep, and the FP64 fit each
-    reaches after the same number of sweeps from the same init."""
+    the tensor resident, next to QRPivProjected with as many samples (its set-up and
+    sweeps) and the plain normal-equation sweep, and the FP64 fit each reaches after the
+    same number of sweeps from the same init."""
     I, R, ns, sweeps = args.size, args.rank, args.samples, 10
     T = synthetic_tensor(I, R, 2024)
     print(f"[bench] tensor {I}^3 generated", file=sys.stderr, flush=True)
@@ -406,6 +407,23 @@ def bench_sampled(args, E) -> dict:
             dt = time.perf_counter() - t0
             res[name] = {"ms_per_sweep": 1e3 * dt / sweeps, "fit_after": fit_of()}
             print(f"[bench] sampled {name}: {res[name]}", file=sys.stderr, flush=True)
+        # QRPivProjected(ns) (normal form, compute_als's default for it): the set-up (a
+        # column-pivoted QR of each 512 x 262144 unfolding) and the sweeps on fixed samples
+        plan.set_cpd(init)
+        plan.synchronize()
+        t0 = time.perf_counter()
+        st = plan.projection_setup(E.QRPivProjected(ns), rng=11)
+        plan.synchronize()
+        setup_ms = 1e3 * (time.perf_counter() - t0)
+        plan.run_projected(st, E.NoCheck(1))                      # warm-up
+        plan.set_cpd(init)
+        plan.synchronize()
+        t0 = time.perf_counter()
+        plan.run_projected(st, E.NoCheck(sweeps))
+        plan.synchronize()
+        res["qrpiv_normal"] = {"setup_ms": setup_ms, "ms_per_sweep": 1e3 * (time.perf_counter() - t0) / sweeps,
+                               "fit_after": fit_of()}
+        print(f"[bench] QRPivProjected normal: {res['qrpiv_normal']}", file=sys.stderr, flush=True)
         plan.set_cpd(init)
         plan.synchronize()
         t0 = time.perf_counter()
