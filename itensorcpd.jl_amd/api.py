@@ -174,11 +174,13 @@ def sparse_sign(d: int, m: int, zeta: int, injective: bool = False, seed: int = 
     z = min(int(zeta), int(d))
     vals = np.empty(m * z)
     rows = np.empty(m * z, dtype=np.int32)
-    colstarts = np.empty(m + 1, dtype=np.int32)
+    # the C column starts are 32-bit, as the reference's; wider embeddings get them here
+    cs32 = np.empty(m + 1, dtype=np.int32) if m * z <= 2**31 - 1 else None
     lib = L.load()
     L.check(lib.itcpd_sparse_sign(int(d), int(m), int(zeta), 1 if injective else 0, C.c_uint64(seed & (2**64 - 1)),
                                   C.c_void_p(vals.ctypes.data), C.c_void_p(rows.ctypes.data),
-                                  C.c_void_p(colstarts.ctypes.data), int(device)))
+                                  None if cs32 is None else C.c_void_p(cs32.ctypes.data), int(device)))
+    colstarts = np.arange(m + 1, dtype=np.int64) * z if cs32 is None else cs32.astype(np.int64)
     return vals, rows, colstarts
 
 

@@ -820,6 +820,8 @@ int itcpd_sparse_sign(int32_t d, int64_t m, int32_t zeta, int32_t injective, uin
     if (d < 1 || m < 1 || zeta < 1) return fail(ITCPD_E_ARG, "sizes must be positive");
     if (zeta > d) zeta = d;
     if (m > ((int64_t)1 << 40) / zeta) return fail(ITCPD_E_ARG, "embedding too large");
+    if (colstarts && m * zeta > (int64_t)INT32_MAX)
+        return fail(ITCPD_E_ARG, "column starts exceed 32-bit offsets (pass colstarts = NULL)");
     DeviceGuard dg(device);
     const size_t nnz = (size_t)m * (size_t)zeta;
     double* dv = nullptr;
