@@ -705,14 +705,17 @@ class ALSPlan:
             seen = np.isin(rmat, p_sk[:t]).any(axis=1)
             indices = np.nonzero(seen)[0].astype(np.int64)
         else:
+            # Omega by rows (columns increasing within a row), then the chosen rows in
+            # pivot order, first occurrence of each column kept
+            flat = rmat.ravel()
+            order = np.argsort(flat, kind="stable")
+            starts = np.searchsorted(flat[order], np.arange(l + 1))
+            cand = np.concatenate([np.unique(order[starts[q]:starts[q + 1]] // s) for q in p_sk[:t]]) \
+                if t > 0 else np.zeros(0, dtype=np.int64)
+            _, first = np.unique(cand, return_index=True)
+            indices = cand[np.sort(first)].astype(np.int64)
             seen = np.zeros(nall, dtype=bool)
-            indices = []
-            for p in p_sk[:t]:
-                for c in np.nonzero((rmat == p).any(axis=1))[0]:
-                    if not seen[c]:
-                        seen[c] = True
-                        indices.append(int(c))
-            indices = np.array(indices, dtype=np.int64)
+            seen[indices] = True
         p_sub, rdiag = self.qr_pivots(mode, indices, source)
         return np.concatenate([indices[p_sub], np.nonzero(~seen)[0]]), rdiag
 
