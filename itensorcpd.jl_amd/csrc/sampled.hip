@@ -879,22 +879,36 @@ __global__ __launch_bounds__(256) void k_qrcp_apply(double* __restrict__ W, int 
     const double* v = W + i + ld * i;
     double* c = W + i + ld * j;
     const double ti = tau[i];
+    // every lane keeps its own rows: row i's new value (lane 0) is broadcast through a
+    // shuffle and the rows below feed the norm recomputation from registers, so no lane
+    // reads another lane's store
+    double ci = 0.0, ssq = 0.0;
     if (ti != 0.0) {
         double s = 0.0;
         for (int k = ln; k < len; k += 64) s += (k == 0 ? 1.0 : v[k]) * c[k];
         const double f = ti * wave_sum(s);
-        for (int k = ln; k < len; k += 64) c[k] -= f * (k == 0 ? 1.0 : v[k]);
+        for (int k = ln; k < len; k += 64) {
+            const double nv = c[k] - f * (k == 0 ? 1.0 : v[k]);
+            c[k] = nv;
+            if (k == 0) ci = nv;
+            else ssq += nv * nv;
+        }
+    } else {
+        for (int k = ln; k < len; k += 64) {
+            const double cv = c[k];
+            if (k == 0) ci = cv;
+            else ssq += cv * cv;
+        }
     }
+    ci = __shfl(ci, 0);
     const double tol3z = 1.0536712127723509e-08;   // sqrt(eps)
     const double v1 = vn1[j];
     if (v1 != 0.0) {
-        double temp = fabs(W[i + ld * j]) / v1;
+        double temp = fabs(ci) / v1;
         temp = fmax(1.0 - temp * temp, 0.0);
         const double q = v1 / vn2[j];
         if (temp * q * q <= tol3z) {
-            double ss = 0.0;
-            for (int k = i + 1 + ln; k < m; k += 64) ss += W[k + ld * j] * W[k + ld * j];
-            ss = wave_sum(ss);
+            const double ss = wave_sum(ssq);
             if (ln == 0) { vn1[j] = sqrt(ss); vn2[j] = vn1[j]; }
         } else if (ln == 0) {
             vn1[j] = v1 * sqrt(temp);
