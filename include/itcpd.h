@@ -224,7 +224,8 @@ int itcpd_sketch_qr_pivots(itcpd_plan* plan, int32_t mode, int32_t source, int64
  * zeta clamped to d.  The reference draws from libc rand(); here every column has
  * its own splitmix64 stream derived from `seed`, generated on the device.  vals,
  * rows (0-based): host, zeta per column, column after column; colstarts (host,
- * nullable, m + 1 entries) = c * zeta. */
+ * nullable, m + 1 entries) = c * zeta, 32-bit as in the reference (an argument error
+ * when m * zeta does not fit; pass NULL then). */
 int itcpd_sparse_sign(int32_t d, int64_t m, int32_t zeta, int32_t injective, uint64_t seed,
                       double* vals, int32_t* rows, int32_t* colstarts, int32_t device);
 
