@@ -721,7 +721,7 @@ class ALSPlan:
 
     def projection_setup(self, alg: "QRPivProjected", trunc_tol: float = 0.01, shuffle_pivots: bool = True,
                          normal: bool = True, injective: bool = False, rng=None, seed=None,
-                         prelim_samples=None, prelim_niter: int = 10) -> "PivotSetup":
+                         prelim_samples=None, prelim_niter: int = 10, prelim_plan=None) -> "PivotSetup":
         """compute_als(::QRPivProjected / ::SEQRCSPivProjected) (optimizers/als_optimizers/
         randomized/qr_lev_score_sampled.jl:1-174): per mode the pivot order (QR-CP of the
         unfolding, or SE-QRCS for alg.random_modes), the effective rank (|r_jj| / max >
@@ -732,16 +732,22 @@ class ALSPlan:
         rmodes = getattr(alg, "random_modes", None) or ()
         rank_sk = getattr(alg, "rank_vect", None)
         ref_pivs, pivots, meffs, samples = [], [], [], []
-        source, start_cpd, prelim = "tensor", None, None
+        source, start_cpd, prelim, src = "tensor", None, None, self
         if isinstance(alg, KSEQRCSPivProjected):
             # compute_als(::KSEQRCSPivProjected) :176-194: 10 LevScoreSampled(10 R) sweeps
-            # from the current CPD, first draws kept; its factors give the pivots and the
-            # run itself starts from the current CPD again
-            start_cpd = self.get_cpd()
-            self.run_sampled(LevScoreSampled(prelim_samples or 10 * self.rank), NoCheck(prelim_niter), normal=True,
-                             stop_resample=0, rng=rng)
-            prelim = self.get_cpd()
+            # from the current CPD (or, with guess_num_levs, on `prelim_plan`, a plan of the
+            # same tensor at the guessed rank holding a fresh random CPD), first draws kept;
+            # those factors give the pivots and the run itself starts from the current CPD
             source = "krp"
+            nsmp = prelim_samples or 10 * self.rank
+            if prelim_plan is None:
+                start_cpd = self.get_cpd()
+                self.run_sampled(LevScoreSampled(nsmp), NoCheck(prelim_niter), normal=True, stop_resample=0, rng=rng)
+                prelim = self.get_cpd()
+            else:
+                src = prelim_plan
+                src.run_sampled(LevScoreSampled(nsmp), NoCheck(prelim_niter), normal=True, stop_resample=0, rng=rng)
+                prelim = src.get_cpd()
         for n in range(N):
             odims = [d for m, d in enumerate(self.dims) if m != n]
             nall = int(np.prod(odims))
@@ -753,9 +759,9 @@ class ALSPlan:
                 s_ = int(round(np.log(m)))
                 sd = int(rng.integers(0, 2**63)) if seed is None else int(seed) + n
                 vals, rows, _ = sparse_sign(l, nall, s_, injective, sd, self.device)
-                p, rdiag = self.seqrcs(n, l, min(s_, l), k_sk, rows, vals, source)
+                p, rdiag = src.seqrcs(n, l, min(s_, l), k_sk, rows, vals, source)
             else:
-                p, rdiag = self.qr_pivots(n, None, source)
+                p, rdiag = src.qr_pivots(n, None, source)
             ref_pivs.append(p.copy())
             a = np.abs(rdiag)
             meff = int((a / a.max() > trunc_tol).sum())
@@ -961,15 +967,31 @@ def update_samples(setup: PivotSetup, new_end, reshuffle: bool = False, new_star
     return PivotSetup(alg, setup.ref_pivs, pivots, setup.effective_ranks, samples, setup.normal, setup.odims)
 
 
+def _guess_plan(target, alg, guess_num_levs, rng):
+    """KSEQRCS with guess_num_levs (qr_lev_score_sampled.jl:188-192): a plan of the target
+    at the guessed rank holding random_CPD(target, guess_num_levs)."""
+    if guess_num_levs is None or not isinstance(alg, KSEQRCSPivProjected):
+        return None
+    plan = ALSPlan(target, int(guess_num_levs))
+    plan.set_cpd(random_CPD(target, int(guess_num_levs), rng))
+    return plan
+
+
 def compute_projection(target, cp: CPD, alg: QRPivProjected, trunc_tol: float = 0.01,
                        shuffle_pivots: bool = True, normal: bool = True, injective: bool = False,
-                       rng=None, seed=None) -> PivotSetup:
+                       rng=None, seed=None, guess_num_levs=None) -> PivotSetup:
     """compute_als for a pivot-based projection on a fresh plan (see
-    ALSPlan.projection_setup)."""
-    with ALSPlan(target, cp.rank) as plan:
-        plan.set_cpd(cp)
-        return plan.projection_setup(alg, trunc_tol=trunc_tol, shuffle_pivots=shuffle_pivots, normal=normal,
-                                     injective=injective, rng=rng, seed=seed)
+    ALSPlan.projection_setup; guess_num_levs as in compute_als(::KSEQRCSPivProjected))."""
+    rng = np.random.default_rng(rng)
+    gp = _guess_plan(target, alg, guess_num_levs, rng)
+    try:
+        with ALSPlan(target, cp.rank) as plan:
+            plan.set_cpd(cp)
+            return plan.projection_setup(alg, trunc_tol=trunc_tol, shuffle_pivots=shuffle_pivots, normal=normal,
+                                         injective=injective, rng=rng, seed=seed, prelim_plan=gp)
+    finally:
+        if gp is not None:
+            gp.close()
 
 
 def optimize_projected(target, cp: CPD, setup: PivotSetup, check, verbose: bool = False) -> CPD:
@@ -983,23 +1005,33 @@ def optimize_projected(target, cp: CPD, setup: PivotSetup, check, verbose: bool 
 
 def als_optimize(target, cp: CPD, alg=None, check=None, maxiter=None, verbose=False, *,
                  normal=None, stop_resample: int = -1, rng=None, shuffle_pivots: bool = True,
-                 trunc_tol: float = 0.01, injective: bool = False, seed=None) -> CPD:
+                 trunc_tol: float = 0.01, injective: bool = False, seed=None, guess_num_levs=None) -> CPD:
     """als_optimizer.jl:15-25 with the defaults of compute_als (:37-55):
     alg = KRPFreeNormal(), check = NoCheck(maxiter or 100).  Returns a new CPD.
     With a ProjectionAlgorithm the projected ALS runs instead: LevScoreSampled draws
     (normal defaults to False, stop_resample, rng seeds the draws); QRPivProjected /
-    SEQRCSPivProjected fix the samples from column-pivoted QRs (normal defaults to
-    True; shuffle_pivots, trunc_tol, injective as compute_als; rng seeds the shuffles
-    and, unless `seed` is given, the embeddings)."""
+    SEQRCSPivProjected / KSEQRCSPivProjected fix the samples from column-pivoted QRs
+    (normal defaults to True; shuffle_pivots, trunc_tol, injective, guess_num_levs as
+    compute_als; rng seeds the shuffles, the draws of KSEQRCS's preliminary run and,
+    unless `seed` is given, the embeddings)."""
     check = NoCheck(100 if maxiter is None else maxiter) if check is None else check
     if isinstance(alg, QRPivProjected):
-        with ALSPlan(target, cp.rank) as plan:
-            plan.set_cpd(cp)
-            st = plan.projection_setup(alg, trunc_tol=trunc_tol, shuffle_pivots=shuffle_pivots,
-                                       normal=True if normal is None else bool(normal), injective=injective,
-                                       rng=rng, seed=seed)
-            plan.run_projected(st, check, verbose=verbose)
-            return plan.get_cpd()
+        rng = np.random.default_rng(rng)
+        gp = _guess_plan(target, alg, guess_num_levs, rng)
+        try:
+            with ALSPlan(target, cp.rank) as plan:
+                plan.set_cpd(cp)
+                st = plan.projection_setup(alg, trunc_tol=trunc_tol, shuffle_pivots=shuffle_pivots,
+                                           normal=True if normal is None else bool(normal), injective=injective,
+                                           rng=rng, seed=seed, prelim_plan=gp)
+                if gp is not None:
+                    gp.close()
+                    gp = None
+                plan.run_projected(st, check, verbose=verbose)
+                return plan.get_cpd()
+        finally:
+            if gp is not None:
+                gp.close()
     if isinstance(alg, ProjectionAlgorithm):
         return _optimize_sampled(target, cp, alg, check, bool(normal), stop_resample, rng, verbose)
     with ALSPlan(target, cp.rank, alg=alg) as plan:

@@ -278,3 +278,22 @@ def test_wide_krp_pivots_match_oracle(itcpd):
         p0, r0 = P.qrcp(P.krp_columns(init.factors, 0))
         assert (p[:6] == p0[:6]).all() and np.abs(r - r0).max() <= 1e-12 * np.abs(r0).max()
         assert sorted(p.tolist()) == list(range(5600))
+
+
+def test_kseqrcs_guess_num_levs(itcpd):
+    """KSEQRCS with guess_num_levs (qr_lev_score_sampled.jl:188-192): the preliminary run
+    starts from a fresh rank-guess CPD (LevScoreSampled(10 R) with the caller's R), and
+    the pivots come from the Khatri-Rao products of that rank-guess CPD."""
+    T = np.random.default_rng(20).standard_normal((12, 11, 10))
+    cp0 = O.random_CPD(T, 5, np.random.default_rng(21))
+    st = itcpd.compute_projection(T, itcpd.CPD(cp0.factors, cp0.lam), itcpd.KSEQRCSPivProjected(1, 80),
+                                  shuffle_pivots=False, rng=4, guess_num_levs=3)
+    assert st.prelim_cpd.rank == 3
+    smp, piv, meff = P.projection_samples(T, 1, 80, shuffle_pivots=False, krp_factors=st.prelim_cpd.factors)
+    assert st.effective_ranks == meff
+    for n in range(3):
+        k = meff[n]
+        assert (st.pivots[n][:k] == piv[n][:k]).all()
+    got = itcpd.als_optimize(T, itcpd.CPD(cp0.factors, cp0.lam), alg=itcpd.KSEQRCSPivProjected(1, 80),
+                             check=itcpd.NoCheck(3), rng=4, guess_num_levs=3)
+    assert all(np.isfinite(f).all() for f in got.factors)
