@@ -868,6 +868,7 @@ __global__ __launch_bounds__(SNT) void k_qrcp_pivot(double* __restrict__ W, int 
 
 // step i: (I - tau v v^T) on every column j > i and its partial-norm update
 // (xLAQP2's downdate, recomputed below sqrt(eps)); a wave per column
+template <int NQ>
 __global__ __launch_bounds__(256) void k_qrcp_apply(double* __restrict__ W, int m, int n, int i,
                                                     const double* __restrict__ tau, double* __restrict__ vn1,
                                                     double* __restrict__ vn2) {
@@ -883,7 +884,29 @@ __global__ __launch_bounds__(256) void k_qrcp_apply(double* __restrict__ W, int 
     // shuffle and the rows below feed the norm recomputation from registers, so no lane
     // reads another lane's store
     double ci = 0.0, ssq = 0.0;
-    if (ti != 0.0) {
+    if (ti != 0.0 && len <= 64 * NQ) {
+        // the column's rows of this lane stay in registers between the two passes
+        double cr[NQ], vr[NQ];
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int k = ln + 64 * q;
+            cr[q] = k < len ? c[k] : 0.0;
+            vr[q] = k < len ? (k == 0 ? 1.0 : v[k]) : 0.0;
+            if (k < len) s += vr[q] * cr[q];
+        }
+        const double f = ti * wave_sum(s);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int k = ln + 64 * q;
+            if (k < len) {
+                const double nv = cr[q] - f * vr[q];
+                c[k] = nv;
+                if (k == 0) ci = nv;
+                else ssq += nv * nv;
+            }
+        }
+    } else if (ti != 0.0) {
         double s = 0.0;
         for (int k = ln; k < len; k += 64) s += (k == 0 ? 1.0 : v[k]) * c[k];
         const double f = ti * wave_sum(s);
@@ -992,9 +1015,15 @@ hipError_t launch_grid_geqp3(double* W, int m, int n, double* tau, double* vwork
                                part_v, part_i);
         hipLaunchKernelGGL(k_qrcp_pivot, dim3(1), dim3(SNT), 0, st, W, m, n, i, part_v, part_i, np, vn1, vn2, jpvt,
                            tau, rdiag);
-        if (i + 1 < n)
-            hipLaunchKernelGGL(k_qrcp_apply, dim3((unsigned)cdiv(n - i - 1, 4)), dim3(256), 0, st, W, m, n, i, tau,
-                               vn1, vn2);
+        // rows per lane kept in registers between the reflector's two passes: 8 up to 512
+        // rows, 32 up to 2048; taller columns take both passes from memory
+        if (i + 1 < n) {
+            const dim3 g((unsigned)cdiv(n - i - 1, 4));
+            if (m <= 512)
+                hipLaunchKernelGGL(k_qrcp_apply<8>, g, dim3(256), 0, st, W, m, n, i, tau, vn1, vn2);
+            else
+                hipLaunchKernelGGL(k_qrcp_apply<32>, g, dim3(256), 0, st, W, m, n, i, tau, vn1, vn2);
+        }
         e = hipGetLastError();
     }
     return e;
