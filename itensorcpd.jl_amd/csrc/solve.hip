@@ -830,10 +830,12 @@ __global__ __launch_bounds__(256) void k_gram_final(const double* __restrict__ g
     if (chk && chk->done) return;
     __shared__ double red[256];
     const int t = threadIdx.x;
-    for (int x = t; x < R * R; x += 256) {
-        double s = 0.0;
-        for (int b = 0; b < nb; ++b) s += gram_part[(int64_t)b * R * R + x];
-        Gout[x] = s;
+    if (gram_part) {   // null: the partials were already summed into Gout by a wider launch
+        for (int x = t; x < R * R; x += 256) {
+            double s = 0.0;
+            for (int b = 0; b < nb; ++b) s += gram_part[(int64_t)b * R * R + x];
+            Gout[x] = s;
+        }
     }
     if (!do_check) return;
     __threadfence_block();
@@ -1211,7 +1213,12 @@ int launch_normalize(itcpd_plan* p, int mode, const double* Xsrc, bool fit_after
                        p->sumsq_part, nsp, rpb, p->A[mode], p->At[mode], p->lam, p->gram_part,
                        p->M, fit_after ? 1 : 0, p->inner_part, 1, p->chk);
     ITCPD_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(256), 0, p->stream, p->gram_part, nb2, R,
+    // the Gram partials summed over the chip (in block order, as k_gram_final would), then
+    // the check on one workgroup
+    int rc;
+    if ((rc = launch_sum_parts(p, MSrc{p->gram_part, nb2, (int64_t)R * R}, (int64_t)R * R, p->G[mode])))
+        return rc;
+    hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(256), 0, p->stream, (const double*)nullptr, nb2, R,
                        p->G[mode], grams_of(p), mode, p->lam, p->inner_part, fit_after ? 1 : 0,
                        p->scal, p->chk, p->fit_trace, p->trace_cap);
     ITCPD_HIP(hipGetLastError());
