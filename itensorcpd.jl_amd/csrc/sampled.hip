@@ -586,59 +586,60 @@ __global__ __launch_bounds__(256) void k_lev_rows(const double* __restrict__ A, 
     probs[i] = s / (double)n;
 }
 
-// The same for n <= 64 with the factor in LDS: k_gram_chol64 (one workgroup) runs
-// k_gram_chol_inv's Cholesky steps on an LDS copy of G and writes U (column-major,
-// zero below the diagonal); k_lev_rows64 solves U^T y = a_i per row (a lane per row,
-// U broadcast from LDS) and p(i) = ||y||^2 / n, the same a_i^T G^-1 a_i / n without
+// The same for n <= 64: k_gram_chol64 factors G = U^T U with the Schur complement in
+// registers and writes U; k_lev_rows64 solves U^T y = a_i per row (a lane per row, U
+// broadcast from LDS) and p(i) = ||y||^2 / n, the same a_i^T G^-1 a_i / n without
 // forming G^-1.
 constexpr int LEV64_LD = 65;
-template <int NT>
-__global__ __launch_bounds__(NT) void k_gram_chol64(const double* __restrict__ G, int n, double* __restrict__ U,
-                                                    int* __restrict__ need, const DevCheck* __restrict__ chk) {
+// One workgroup of 256 threads, the Schur complement in registers: thread (w, l)
+// holds rows 16w..16w+15 of column l.  Step j: the wave owning row j hands it over
+// through LDS (one barrier), u_l = S(j, l) / sqrt(S(j, j)) and every thread applies
+// the rank-1 update to its 16 entries; the pivot test is k_gram_chol_inv's (stop when
+// S(j, j) <= n eps max(diag G)).  Writes U (column-major, zero below the diagonal).
+__global__ __launch_bounds__(256) void k_gram_chol64(const double* __restrict__ G, int n, double* __restrict__ U,
+                                                     int* __restrict__ need, const DevCheck* __restrict__ chk) {
     if (chk->done) return;
-    __shared__ double S[64 * LEV64_LD];
-    __shared__ double redv[NT];
-    __shared__ int s_bad;
-    const int t = threadIdx.x;
-    for (int x = t; x < n * n; x += NT) S[(x % n) * LEV64_LD + x / n] = G[x];
+    __shared__ double rowb[2][64];
+    __shared__ double redv[256];
+    const int t = threadIdx.x, l = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    d16 sr;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int m = 16 * w + i;
+        sr[i] = (m < n && l < n) ? G[m + (int64_t)n * l] : 0.0;
+    }
     double dmax = 0.0;
-    for (int j = t; j < n; j += NT) dmax = fmax(dmax, G[j + (int64_t)n * j]);
+    for (int j = t; j < n; j += 256) dmax = fmax(dmax, G[j + (int64_t)n * j]);
     redv[t] = dmax;
-    if (t == 0) s_bad = 0;
     __syncthreads();
-    for (int o = NT / 2; o > 0; o >>= 1) {
+    for (int o = 128; o > 0; o >>= 1) {
         if (t < o) redv[t] = fmax(redv[t], redv[t + o]);
         __syncthreads();
     }
     const double thr = (double)n * EPS64 * redv[0];
+    bool bad = false;
     for (int j = 0; j < n; ++j) {
-        const double d = S[j * LEV64_LD + j];
+        double* rb = rowb[j & 1];
+        if (w == (j >> 4)) rb[l] = sr[j & 15];
+        __syncthreads();
+        const double d = rb[j];
         if (!(d > thr)) {
-            if (t == 0) s_bad = 1;
-            __syncthreads();
+            bad = true;
             break;
         }
         const double ujj = sqrt(d), inv = 1.0 / ujj;
-        __syncthreads();
-        for (int k = j + 1 + t; k < n; k += NT) S[j * LEV64_LD + k] *= inv;
-        if (t == 0) S[j * LEV64_LD + j] = ujj;
-        __syncthreads();
-        const int r = n - j - 1;
-        for (int x = t; x < r * r; x += NT) {
-            const int a = j + 1 + x % r, b = j + 1 + x / r;
-            if (a <= b) S[a * LEV64_LD + b] -= S[j * LEV64_LD + a] * S[j * LEV64_LD + b];
+        const double ul = rb[l] * inv;
+        if (w == 0 && l < n) U[j + (int64_t)n * l] = l > j ? ul : (l == j ? ujj : 0.0);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int m = 16 * w + i;
+            const double um = rb[m] * inv;
+            if (m > j && l > j) sr[i] -= um * ul;
         }
-        __syncthreads();
     }
-    if (s_bad) {
-        if (t == 0) *need = 1;
-        return;
-    }
-    for (int x = t; x < n * n; x += NT) {
-        const int a = x % n, b = x / n;
-        U[x] = a <= b ? S[a * LEV64_LD + b] : 0.0;
-    }
-    if (t == 0) *need = 0;
+    // row j of U (with its zeros left of the diagonal) was written at step j
+    if (t == 0) *need = bad ? 1 : 0;
 }
 
 __global__ __launch_bounds__(64) void k_lev_rows64(const double* __restrict__ A, int m, int n,
@@ -1103,7 +1104,7 @@ int launch_leverage_scores(itcpd_plan* p, int mode, double* probs_dev, double** 
                        p->A[mode], p->A[mode], (int64_t)m, n, n, G, p->chk);
     ITCPD_HIP(hipGetLastError());
     if (n <= 64) {
-        hipLaunchKernelGGL(k_gram_chol64<256>, dim3(1), dim3(256), 0, p->stream, G, n, Ui, need, p->chk);
+        hipLaunchKernelGGL(k_gram_chol64, dim3(1), dim3(256), 0, p->stream, G, n, Ui, need, p->chk);
         ITCPD_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_lev_rows64, dim3((unsigned)cdiv(m, 64)), dim3(64), 0, p->stream, p->A[mode], m, n, Ui,
                            need, probs, p->chk);
